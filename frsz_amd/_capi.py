@@ -1,0 +1,78 @@
+"""ctypes binding of the C ABI in include/frsz2_capi.h (libfrsz2_hip.so).
+
+The library is the product: every codec call below runs a HIP kernel.  If
+the library has not been built, importing this module raises immediately;
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfrsz2_hip.so")
+
+FLOAT32 = 0
+FLOAT64 = 1
+
+OK, EDTYPE, EPARAM, ECAPACITY, EARG, EHIP = range(6)
+
+
+class Frsz2Error(RuntimeError):
+    """A non-zero status of the C ABI; ``code`` follows the reference plugin
+    (1 unsupported dtype, 2 unsupported bits / block size) plus 3..5."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"frsz2 error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: the frsz2 HIP library has not been built "
+            "(run __graft_entry__.build()); there is no CPU fallback")
+    return ctypes.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_u64 = ctypes.c_uint64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+lib.frsz2_last_error.restype = ctypes.c_char_p
+lib.frsz2_last_error.argtypes = []
+lib.frsz2_version.restype = ctypes.c_char_p
+lib.frsz2_version.argtypes = []
+lib.frsz2_check.restype = _int
+lib.frsz2_check.argtypes = [_int, _int, _u64]
+lib.frsz2_compressed_size.restype = _int
+lib.frsz2_compressed_size.argtypes = [_int, _int, _u64, _u64, _u64p]
+lib.frsz2_compress.restype = _int
+lib.frsz2_compress.argtypes = [_int, _int, _u64, _vp, _u64, _vp, _u64, _vp]
+lib.frsz2_decompress.restype = _int
+lib.frsz2_decompress.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp, _vp]
+lib.frsz2_gather.restype = _int
+lib.frsz2_gather.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp, _u64, _vp, _vp]
+lib.frsz2_compress_host.restype = _int
+lib.frsz2_compress_host.argtypes = [_int, _int, _u64, _vp, _u64, _vp, _u64]
+lib.frsz2_decompress_host.restype = _int
+lib.frsz2_decompress_host.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp]
+lib.frsz2_gather_host.restype = _int
+lib.frsz2_gather_host.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp, _u64, _vp]
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise Frsz2Error(rc, lib.frsz2_last_error().decode())
+
+
+def last_error() -> str:
+    return lib.frsz2_last_error().decode()
+
+
+def version() -> str:
+    return lib.frsz2_version().decode()

@@ -1,0 +1,189 @@
+// Error state and the host-memory entry points of libfrsz2_hip.so.
+//
+// The host variants keep the host-in / host-out meaning of the reference's
+// CPU codec (include/frsz2.hpp:1181-1249) and LibPressio plugin
+// (src/frsz2.cc:152-186): they copy the input to device memory, run the same
+// HIP kernels as the device entry points on a private stream, and copy the
+// result back before returning.
+#include "frsz2_capi.h"
+#include "frsz2_device.hpp"
+#include "frsz2_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace frsz2 {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int
+set_error(int code, const std::string& msg)
+{
+  g_error = msg;
+  return code;
+}
+
+void
+clear_error()
+{
+  g_error.clear();
+}
+
+namespace {
+
+// device buffers + a non-blocking stream released on scope exit
+struct staging
+{
+  hipStream_t s = nullptr;
+  void* bufs[3] = { nullptr, nullptr, nullptr };
+  ~staging()
+  {
+    for (void* b : bufs) {
+      if (b) {
+        (void)hipFree(b);
+      }
+    }
+    if (s) {
+      (void)hipStreamDestroy(s);
+    }
+  }
+  hipError_t init() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+  hipError_t alloc(int i, std::uint64_t bytes) { return hipMalloc(&bufs[i], bytes ? bytes : 16); }
+};
+
+#define FRSZ2_HIP_TRY(expr, what)                                                                            \
+  do {                                                                                                       \
+    const hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                                    \
+      return hip_status(e_, what);                                                                           \
+  } while (0)
+
+} // namespace
+
+} // namespace frsz2
+
+extern "C" {
+
+const char*
+frsz2_last_error(void)
+{
+  return frsz2::g_error.c_str();
+}
+
+int
+frsz2_compress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t n, void* h_out,
+                    uint64_t out_capacity)
+{
+  using namespace frsz2;
+  clear_error();
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(bits, block, n);
+  const std::uint64_t total = L.total_bytes();
+  if (out_capacity < total) {
+    return set_error(FRSZ2_ECAPACITY,
+                     "output capacity " + std::to_string(out_capacity) + " < stream size " + std::to_string(total));
+  }
+  if (!h_out || (n > 0 && !h_in)) {
+    return set_error(FRSZ2_EARG, "null pointer");
+  }
+  const std::uint64_t in_bytes = n * (dtype == FRSZ2_FLOAT32 ? 4u : 8u);
+  staging st;
+  FRSZ2_HIP_TRY(st.init(), "hipStreamCreate");
+  FRSZ2_HIP_TRY(st.alloc(0, in_bytes), "hipMalloc");
+  FRSZ2_HIP_TRY(st.alloc(1, total), "hipMalloc");
+  if (in_bytes) {
+    FRSZ2_HIP_TRY(hipMemcpyAsync(st.bufs[0], h_in, in_bytes, hipMemcpyHostToDevice, st.s), "hipMemcpyAsync");
+  }
+  rc = compress_device(dtype, bits, block, st.bufs[0], n, st.bufs[1], total, st.s);
+  if (rc) {
+    return rc;
+  }
+  FRSZ2_HIP_TRY(hipMemcpyAsync(h_out, st.bufs[1], total, hipMemcpyDeviceToHost, st.s), "hipMemcpyAsync");
+  FRSZ2_HIP_TRY(hipStreamSynchronize(st.s), "hipStreamSynchronize");
+  return FRSZ2_OK;
+}
+
+int
+frsz2_decompress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
+                      void* h_out)
+{
+  using namespace frsz2;
+  clear_error();
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(bits, block, n);
+  const std::uint64_t total = L.total_bytes();
+  if (in_bytes < total) {
+    return set_error(FRSZ2_ECAPACITY,
+                     "input stream of " + std::to_string(in_bytes) + " bytes < stream size " + std::to_string(total));
+  }
+  if (!h_in || (n > 0 && !h_out)) {
+    return set_error(FRSZ2_EARG, "null pointer");
+  }
+  const std::uint64_t out_bytes = n * (dtype == FRSZ2_FLOAT32 ? 4u : 8u);
+  staging st;
+  FRSZ2_HIP_TRY(st.init(), "hipStreamCreate");
+  FRSZ2_HIP_TRY(st.alloc(0, total), "hipMalloc");
+  FRSZ2_HIP_TRY(st.alloc(1, out_bytes), "hipMalloc");
+  FRSZ2_HIP_TRY(hipMemcpyAsync(st.bufs[0], h_in, total, hipMemcpyHostToDevice, st.s), "hipMemcpyAsync");
+  rc = decompress_device(dtype, bits, block, st.bufs[0], total, n, st.bufs[1], st.s);
+  if (rc) {
+    return rc;
+  }
+  if (out_bytes) {
+    FRSZ2_HIP_TRY(hipMemcpyAsync(h_out, st.bufs[1], out_bytes, hipMemcpyDeviceToHost, st.s), "hipMemcpyAsync");
+  }
+  FRSZ2_HIP_TRY(hipStreamSynchronize(st.s), "hipStreamSynchronize");
+  return FRSZ2_OK;
+}
+
+int
+frsz2_gather_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
+                  const uint64_t* h_idx, uint64_t m, void* h_out)
+{
+  using namespace frsz2;
+  clear_error();
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(bits, block, n);
+  const std::uint64_t total = L.total_bytes();
+  if (in_bytes < total) {
+    return set_error(FRSZ2_ECAPACITY,
+                     "input stream of " + std::to_string(in_bytes) + " bytes < stream size " + std::to_string(total));
+  }
+  if (m == 0) {
+    return FRSZ2_OK;
+  }
+  if (!h_in || !h_idx || !h_out) {
+    return set_error(FRSZ2_EARG, "null pointer");
+  }
+  const std::uint64_t vs = dtype == FRSZ2_FLOAT32 ? 4u : 8u;
+  staging st;
+  FRSZ2_HIP_TRY(st.init(), "hipStreamCreate");
+  FRSZ2_HIP_TRY(st.alloc(0, total), "hipMalloc");
+  FRSZ2_HIP_TRY(st.alloc(1, m * 8), "hipMalloc");
+  FRSZ2_HIP_TRY(st.alloc(2, m * vs), "hipMalloc");
+  FRSZ2_HIP_TRY(hipMemcpyAsync(st.bufs[0], h_in, total, hipMemcpyHostToDevice, st.s), "hipMemcpyAsync");
+  FRSZ2_HIP_TRY(hipMemcpyAsync(st.bufs[1], h_idx, m * 8, hipMemcpyHostToDevice, st.s), "hipMemcpyAsync");
+  rc = gather_device(dtype, bits, block, st.bufs[0], total, n, static_cast<const std::uint64_t*>(st.bufs[1]), m,
+                     st.bufs[2], st.s);
+  if (rc) {
+    return rc;
+  }
+  FRSZ2_HIP_TRY(hipMemcpyAsync(h_out, st.bufs[2], m * vs, hipMemcpyDeviceToHost, st.s), "hipMemcpyAsync");
+  FRSZ2_HIP_TRY(hipStreamSynchronize(st.s), "hipStreamSynchronize");
+  return FRSZ2_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,469 @@
+// Plain single-array FRSZ2 kernels for gfx950 and their C-ABI entry points.
+//
+//   compress    k_compress_tile<T,U,VPT,PACKED>   one 256-thread workgroup per
+//               tile of 256*VPT values (power-of-two B <= tile), XCD-aware
+//               tile order; k_compress_block for any other B (one thread per
+//               exponent block) and for pointers that are not 16-byte aligned.
+//   decompress  k_decompress_tile<...> / k_decompress_elem (one thread per value)
+//   gather      k_gather (one thread per index)
+//
+// The arithmetic follows include/frsz2.hpp of robertu94/frsz; the value-level
+// rules live in include/frsz2_device.hpp and the tile-level ones in tile.hpp.
+#include "frsz2_capi.h"
+#include "frsz2_device.hpp"
+#include "frsz2_internal.hpp"
+#include "tile.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace frsz2 {
+
+using tile::tplan;
+using tile::WG;
+
+// ---------------------------------------------------------------- kernels
+
+template<typename T, int U, int VPT, bool PACKED>
+__global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw* __restrict__ in,
+                                                      std::uint8_t* __restrict__ out,
+                                                      tplan p,
+                                                      std::uint32_t ntiles)
+{
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
+  __shared__ std::uint32_t wred[WG / tile::WAVE];
+  const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
+  tile::compress_tile<T, U, VPT, PACKED>(in, out, p, t, lds, wred);
+  if (t == 0) {
+    tile::zero_pad(out, p.nb, U, p.R, static_cast<int>(threadIdx.x), WG);
+  }
+}
+
+template<typename T, int U, int VPT, bool PACKED>
+__global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __restrict__ in,
+                                                        typename fmt<T>::uw* __restrict__ out,
+                                                        tplan p,
+                                                        std::uint32_t ntiles)
+{
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
+  const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
+  tile::decompress_tile<T, U, VPT, PACKED>(in, out, p, t, lds);
+}
+
+// One thread per exponent block, any B; builds each payload word from the
+// codes that overlap it and writes all CB bytes of the block.
+template<typename T, int U>
+__global__ __launch_bounds__(WG) void k_compress_block(const typename fmt<T>::uw* __restrict__ in,
+                                                       std::uint8_t* __restrict__ out,
+                                                       layout L)
+{
+  using uw = typename fmt<T>::uw;
+  using wt = typename word<U>::type;
+  const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (gid < 256) {
+    tile::zero_pad(out, L.nb, U, L.R, static_cast<int>(gid), 256);
+  }
+  if (gid >= L.nb) {
+    return;
+  }
+  const std::uint64_t k = gid;
+  const std::uint64_t first = k * L.B;
+  const std::uint64_t len = (L.n - first) < L.B ? (L.n - first) : L.B;
+  std::uint32_t m = 0;
+  for (std::uint64_t i = 0; i < len; ++i) {
+    const std::uint32_t key = max_key<T>(in[first + i]);
+    m = m > key ? m : key;
+  }
+  const int E = block_exponent_from_key<T, U>(m);
+  tile::store_exponent<U>(out, k, E);
+  wt* words = reinterpret_cast<wt*>(out + L.R + k * L.CB);
+  const std::uint64_t nwords = L.CB / (U / 8);
+  const std::uint64_t bits = static_cast<std::uint64_t>(L.bits);
+  for (std::uint64_t w = 0; w < nwords; ++w) {
+    const std::uint64_t lo = w * U;
+    const std::uint64_t hi = lo + U;
+    std::uint64_t j0 = lo / bits;
+    std::uint64_t j1 = (hi + bits - 1) / bits;
+    if (j1 > len) {
+      j1 = len;
+    }
+    std::uint64_t acc = 0;
+    for (std::uint64_t j = j0; j < j1; ++j) {
+      const std::uint64_t c = static_cast<std::uint64_t>(encode_value<T>(in[first + j], E, L.bits));
+      const std::int64_t sh = static_cast<std::int64_t>(j * bits) - static_cast<std::int64_t>(lo);
+      acc |= sh >= 0 ? (c << sh) : (c >> (-sh));
+    }
+    words[w] = static_cast<wt>(acc);
+  }
+}
+
+template<typename T, int U>
+__global__ __launch_bounds__(WG) void k_decompress_elem(const std::uint8_t* __restrict__ in,
+                                                        T* __restrict__ out,
+                                                        layout L)
+{
+  const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (i < L.n) {
+    out[i] = decode_element<T, U>(in, L, i);
+  }
+}
+
+template<typename T, int U>
+__global__ __launch_bounds__(WG) void k_gather(const std::uint8_t* __restrict__ in,
+                                               layout L,
+                                               const std::uint64_t* __restrict__ idx,
+                                               std::uint64_t m,
+                                               T* __restrict__ out)
+{
+  const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (i < m) {
+    const std::uint64_t j = idx[i];
+    out[i] = j < L.n ? decode_element<T, U>(in, L, j) : T(0);
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+
+namespace {
+
+int
+ilog2_exact(std::uint64_t B)
+{
+  if (B == 0 || (B & (B - 1)) != 0) {
+    return -1;
+  }
+  int l = 0;
+  while ((std::uint64_t{ 1 } << l) < B) {
+    ++l;
+  }
+  return l;
+}
+
+inline unsigned
+grid_for(std::uint64_t work)
+{
+  return static_cast<unsigned>((work + WG - 1) / WG);
+}
+
+template<typename T, int U, int VPT>
+hipError_t
+launch_compress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
+{
+  constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
+  tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0 };
+  std::uint64_t nt = (L.n + TILE - 1) / TILE;
+  if (nt == 0) {
+    nt = 1;
+  }
+  const auto* x = static_cast<const typename fmt<T>::uw*>(in);
+  auto* o = static_cast<std::uint8_t*>(out);
+  if (L.bits == U) {
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+  } else {
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+  }
+  return hipGetLastError();
+}
+
+template<typename T, int U, int VPT>
+hipError_t
+launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
+{
+  constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
+  tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0 };
+  const std::uint64_t nt = (L.n + TILE - 1) / TILE;
+  if (nt == 0) {
+    return hipSuccess;
+  }
+  const auto* x = static_cast<const std::uint8_t*>(in);
+  auto* o = static_cast<typename fmt<T>::uw*>(out);
+  if (L.bits == U) {
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+  } else {
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+  }
+  return hipGetLastError();
+}
+
+template<typename T, int U>
+hipError_t
+launch_compress_generic(const void* in, void* out, const layout& L, hipStream_t s)
+{
+  std::uint64_t threads = L.nb < 256 ? 256 : L.nb;
+  hipLaunchKernelGGL((k_compress_block<T, U>), dim3(grid_for(threads)), dim3(WG), 0, s,
+                     static_cast<const typename fmt<T>::uw*>(in), static_cast<std::uint8_t*>(out), L);
+  return hipGetLastError();
+}
+
+template<typename T, int U>
+hipError_t
+launch_decompress_generic(const void* in, void* out, const layout& L, hipStream_t s)
+{
+  if (L.n == 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL((k_decompress_elem<T, U>), dim3(grid_for(L.n)), dim3(WG), 0, s,
+                     static_cast<const std::uint8_t*>(in), static_cast<T*>(out), L);
+  return hipGetLastError();
+}
+
+template<typename T, int U>
+hipError_t
+launch_gather(const void* in, const layout& L, const std::uint64_t* idx, std::uint64_t m, void* out, hipStream_t s)
+{
+  if (m == 0) {
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL((k_gather<T, U>), dim3(grid_for(m)), dim3(WG), 0, s, static_cast<const std::uint8_t*>(in), L,
+                     idx, m, static_cast<T*>(out));
+  return hipGetLastError();
+}
+
+// values per thread of the tiled kernels, per (dtype, U)
+template<typename T, int U>
+struct vpt_of
+{
+  static constexpr int value = 4;
+};
+
+template<typename T, int U>
+hipError_t
+run_compress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+{
+  constexpr int VPT = vpt_of<T, U>::value;
+  if (tiled && (static_cast<std::uint64_t>(WG) * VPT) >= L.B) {
+    return launch_compress_tiled<T, U, VPT>(in, out, L, lb, s);
+  }
+  return launch_compress_generic<T, U>(in, out, L, s);
+}
+
+template<typename T, int U>
+hipError_t
+run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+{
+  constexpr int VPT = vpt_of<T, U>::value;
+  if (tiled && (static_cast<std::uint64_t>(WG) * VPT) >= L.B) {
+    return launch_decompress_tiled<T, U, VPT>(in, out, L, lb, s);
+  }
+  return launch_decompress_generic<T, U>(in, out, L, s);
+}
+
+template<template<typename, int> class Op, typename... A>
+hipError_t
+by_type(int dtype, int U, A&&... a)
+{
+  if (dtype == FRSZ2_FLOAT32) {
+    switch (U) {
+      case 8:
+        return Op<float, 8>::run(a...);
+      case 16:
+        return Op<float, 16>::run(a...);
+      default:
+        return Op<float, 32>::run(a...);
+    }
+  }
+  switch (U) {
+    case 8:
+      return Op<double, 8>::run(a...);
+    case 16:
+      return Op<double, 16>::run(a...);
+    case 32:
+      return Op<double, 32>::run(a...);
+    default:
+      return Op<double, 64>::run(a...);
+  }
+}
+
+template<typename T, int U>
+struct compress_op
+{
+  static hipError_t run(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+  {
+    return run_compress<T, U>(in, out, L, tiled, lb, s);
+  }
+};
+
+template<typename T, int U>
+struct decompress_op
+{
+  static hipError_t run(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+  {
+    return run_decompress<T, U>(in, out, L, tiled, lb, s);
+  }
+};
+
+template<typename T, int U>
+struct gather_op
+{
+  static hipError_t run(const void* in, const layout& L, const std::uint64_t* idx, std::uint64_t m, void* out,
+                        hipStream_t s)
+  {
+    return launch_gather<T, U>(in, L, idx, m, out, s);
+  }
+};
+
+inline bool
+aligned(const void* p, std::uintptr_t a)
+{
+  return (reinterpret_cast<std::uintptr_t>(p) & (a - 1)) == 0;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- internal API
+
+int
+check_params(int dtype, int bits, std::uint64_t block)
+{
+  if (dtype != FRSZ2_FLOAT32 && dtype != FRSZ2_FLOAT64) {
+    return set_error(FRSZ2_EDTYPE, "unsupported dtype");
+  }
+  const int W = dtype == FRSZ2_FLOAT32 ? 32 : 64;
+  if (bits < 4 || bits > W) {
+    return set_error(FRSZ2_EPARAM, "Unsupported number of bits: " + std::to_string(bits));
+  }
+  if (block == 0 || block > FRSZ2_MAX_BLOCK) {
+    return set_error(FRSZ2_EPARAM, "Unsupported exponent block size: " + std::to_string(block));
+  }
+  return FRSZ2_OK;
+}
+
+int
+hip_status(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) {
+    return set_error(FRSZ2_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+  return FRSZ2_OK;
+}
+
+int
+compress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t n, void* d_out,
+                std::uint64_t out_capacity, hipStream_t s)
+{
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(bits, block, n);
+  if (out_capacity < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, "output capacity " + std::to_string(out_capacity) + " < stream size " +
+                                        std::to_string(L.total_bytes()));
+  }
+  const std::uintptr_t vs = dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  if (!d_out || (n > 0 && !d_in) || !aligned(d_in, vs) || !aligned(d_out, 8)) {
+    return set_error(FRSZ2_EARG, "null or misaligned pointer");
+  }
+  const int lb = ilog2_exact(block);
+  const bool tiled = lb >= 0 && aligned(d_in, 16) && aligned(d_out, 16);
+  return hip_status(by_type<compress_op>(dtype, L.U, d_in, d_out, L, tiled, lb, s), "compress launch");
+}
+
+int
+decompress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t in_bytes,
+                  std::uint64_t n, void* d_out, hipStream_t s)
+{
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(bits, block, n);
+  if (in_bytes < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, "input stream of " + std::to_string(in_bytes) + " bytes < stream size " +
+                                        std::to_string(L.total_bytes()));
+  }
+  const std::uintptr_t vs = dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  if (!d_in || (n > 0 && !d_out) || !aligned(d_in, 8) || !aligned(d_out, vs)) {
+    return set_error(FRSZ2_EARG, "null or misaligned pointer");
+  }
+  const int lb = ilog2_exact(block);
+  const bool tiled = lb >= 0 && aligned(d_in, 16) && aligned(d_out, 16);
+  return hip_status(by_type<decompress_op>(dtype, L.U, d_in, d_out, L, tiled, lb, s), "decompress launch");
+}
+
+int
+gather_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t in_bytes, std::uint64_t n,
+              const std::uint64_t* d_idx, std::uint64_t m, void* d_out, hipStream_t s)
+{
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(bits, block, n);
+  if (in_bytes < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, "input stream of " + std::to_string(in_bytes) + " bytes < stream size " +
+                                        std::to_string(L.total_bytes()));
+  }
+  const std::uintptr_t vs = dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  if (m > 0 && (!d_in || !d_idx || !d_out || !aligned(d_in, 8) || !aligned(d_out, vs) || !aligned(d_idx, 8))) {
+    return set_error(FRSZ2_EARG, "null or misaligned pointer");
+  }
+  return hip_status(by_type<gather_op>(dtype, L.U, d_in, L, d_idx, m, d_out, s), "gather launch");
+}
+
+} // namespace frsz2
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char*
+frsz2_version(void)
+{
+  return "frsz2-mi355x 0.1.0 (stream format of frsz2 plugin 0.0.2)";
+}
+
+int
+frsz2_check(int dtype, int bits, uint64_t block)
+{
+  frsz2::clear_error();
+  return frsz2::check_params(dtype, bits, block);
+}
+
+int
+frsz2_compressed_size(int dtype, int bits, uint64_t block, uint64_t n, uint64_t* bytes)
+{
+  frsz2::clear_error();
+  const int rc = frsz2::check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  if (!bytes) {
+    return frsz2::set_error(FRSZ2_EARG, "null output pointer");
+  }
+  *bytes = frsz2::layout::make(bits, block, n).total_bytes();
+  return FRSZ2_OK;
+}
+
+int
+frsz2_compress(int dtype, int bits, uint64_t block, const void* d_in, uint64_t n, void* d_out,
+               uint64_t out_capacity, frsz2_stream_t stream)
+{
+  frsz2::clear_error();
+  return frsz2::compress_device(dtype, bits, block, d_in, n, d_out, out_capacity,
+                                static_cast<hipStream_t>(stream));
+}
+
+int
+frsz2_decompress(int dtype, int bits, uint64_t block, const void* d_in, uint64_t in_bytes, uint64_t n,
+                 void* d_out, frsz2_stream_t stream)
+{
+  frsz2::clear_error();
+  return frsz2::decompress_device(dtype, bits, block, d_in, in_bytes, n, d_out, static_cast<hipStream_t>(stream));
+}
+
+int
+frsz2_gather(int dtype, int bits, uint64_t block, const void* d_in, uint64_t in_bytes, uint64_t n,
+             const uint64_t* d_idx, uint64_t m, void* d_out, frsz2_stream_t stream)
+{
+  frsz2::clear_error();
+  return frsz2::gather_device(dtype, bits, block, d_in, in_bytes, n, d_idx, m, d_out,
+                              static_cast<hipStream_t>(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,553 @@
+// Tile-level FRSZ2 compress / decompress for gfx950, shared by the plain
+// single-array kernels and the persistent batch kernel.
+//
+// A tile is WG*VPT consecutive values (WG = 256 threads, VPT values per
+// thread), i.e. TB = tile/B whole exponent blocks; B is a power of two no
+// larger than the tile.  Thread t owns values [t*VPT, t*VPT+VPT) of the tile,
+// loaded and stored with 128-bit accesses.
+//
+// compress:  per-value max key (biased exponent, NaN -> 0) -> thread max ->
+//            xor-butterfly max over the B/VPT lanes of one block (plus an LDS
+//            step across waves when a block spans more than one wave) ->
+//            block exponent -> per-value code.
+//   bits == U: codes are U-bit words, payload word i = value i; stored
+//            straight from registers.
+//   bits <  U: codes are OR-ed into an LDS image of the tile's payload
+//            (ds_or_b32 at their bit offsets), which is then written out with
+//            128-bit stores.  For power-of-two B the tile payload TB*CB is a
+//            multiple of 16 bytes and starts at a 16-byte aligned offset.
+// decompress: the reverse; for bits < U the tile payload is staged into LDS
+//            with 128-bit loads and codes are extracted with v_alignbit.
+#pragma once
+
+#include "frsz2_device.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace frsz2 {
+namespace tile {
+
+constexpr int WG = 256;
+constexpr int WAVE = 64;
+
+// Per-array parameters of the tile path.
+struct tplan
+{
+  std::uint64_t n;
+  std::uint64_t nb;
+  std::uint64_t R;
+  std::uint32_t CB;
+  int bits;
+  int lb; // B = 1 << lb
+  int pad_;
+};
+
+typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// LDS dwords a packed tile needs (payload image plus 3 dwords of slack for
+// the funnel reads / ORs that touch the dword after a code's last one)
+template<int VPT>
+constexpr int lds_dwords()
+{
+  return WG * VPT * 2 + 4;
+}
+
+template<typename T, int VPT>
+__device__ __forceinline__ void
+load_values(const typename fmt<T>::uw* __restrict__ p, typename fmt<T>::uw (&x)[VPT])
+{
+  constexpr int NV = VPT * static_cast<int>(sizeof(T)) / 16;
+  const u32x4* q = reinterpret_cast<const u32x4*>(p);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    const u32x4 d = __builtin_nontemporal_load(q + c);
+    if (sizeof(T) == 4) {
+      x[4 * c + 0] = d.x;
+      x[4 * c + 1] = d.y;
+      x[4 * c + 2] = d.z;
+      x[4 * c + 3] = d.w;
+    } else {
+      x[2 * c + 0] = static_cast<typename fmt<T>::uw>(d.x) | (static_cast<std::uint64_t>(d.y) << 32);
+      x[2 * c + 1] = static_cast<typename fmt<T>::uw>(d.z) | (static_cast<std::uint64_t>(d.w) << 32);
+    }
+  }
+}
+
+template<typename T, int VPT>
+__device__ __forceinline__ void
+store_values(typename fmt<T>::uw* __restrict__ p, const typename fmt<T>::uw (&y)[VPT])
+{
+  constexpr int NV = VPT * static_cast<int>(sizeof(T)) / 16;
+  u32x4* q = reinterpret_cast<u32x4*>(p);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    u32x4 d;
+    if (sizeof(T) == 4) {
+      d.x = static_cast<std::uint32_t>(y[4 * c + 0]);
+      d.y = static_cast<std::uint32_t>(y[4 * c + 1]);
+      d.z = static_cast<std::uint32_t>(y[4 * c + 2]);
+      d.w = static_cast<std::uint32_t>(y[4 * c + 3]);
+    } else {
+      d.x = static_cast<std::uint32_t>(y[2 * c + 0]);
+      d.y = static_cast<std::uint32_t>(static_cast<std::uint64_t>(y[2 * c + 0]) >> 32);
+      d.z = static_cast<std::uint32_t>(y[2 * c + 1]);
+      d.w = static_cast<std::uint32_t>(static_cast<std::uint64_t>(y[2 * c + 1]) >> 32);
+    }
+    __builtin_nontemporal_store(d, q + c);
+  }
+}
+
+// VPT U-bit codes <-> VPT*U/32 dwords (little-endian word order)
+template<int U, int VPT, typename UW>
+__device__ __forceinline__ void
+pack_words(const UW (&code)[VPT], std::uint32_t (&w)[VPT * U / 32 > 0 ? VPT * U / 32 : 1])
+{
+  if constexpr (U == 64) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      w[2 * v] = static_cast<std::uint32_t>(code[v]);
+      w[2 * v + 1] = static_cast<std::uint32_t>(static_cast<std::uint64_t>(code[v]) >> 32);
+    }
+  } else {
+    constexpr int PER = 32 / U;
+#pragma unroll
+    for (int d = 0; d < VPT / PER; ++d) {
+      std::uint32_t acc = 0;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        acc |= static_cast<std::uint32_t>(code[d * PER + i]) << (i * U);
+      }
+      w[d] = acc;
+    }
+  }
+}
+
+template<int U, int VPT, typename UW>
+__device__ __forceinline__ void
+unpack_words(const std::uint32_t (&w)[VPT * U / 32 > 0 ? VPT * U / 32 : 1], UW (&code)[VPT])
+{
+  if constexpr (U == 64) {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      code[v] = static_cast<UW>(w[2 * v]) | static_cast<UW>(static_cast<std::uint64_t>(w[2 * v + 1]) << 32);
+    }
+  } else {
+    constexpr int PER = 32 / U;
+    constexpr std::uint32_t M = U == 32 ? 0xffffffffu : ((1u << U) - 1u);
+#pragma unroll
+    for (int d = 0; d < VPT / PER; ++d) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        code[d * PER + i] = static_cast<UW>((w[d] >> (i * U)) & M);
+      }
+    }
+  }
+}
+
+template<int N>
+__device__ __forceinline__ void
+store_dwords(std::uint8_t* __restrict__ dst, const std::uint32_t (&w)[N])
+{
+  if constexpr (N % 4 == 0) {
+    u32x4* q = reinterpret_cast<u32x4*>(dst);
+#pragma unroll
+    for (int c = 0; c < N / 4; ++c) {
+      u32x4 d;
+      d.x = w[4 * c];
+      d.y = w[4 * c + 1];
+      d.z = w[4 * c + 2];
+      d.w = w[4 * c + 3];
+      __builtin_nontemporal_store(d, q + c);
+    }
+  } else if constexpr (N == 2) {
+    typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    u32x2 d;
+    d.x = w[0];
+    d.y = w[1];
+    __builtin_nontemporal_store(d, reinterpret_cast<u32x2*>(dst));
+  } else {
+    __builtin_nontemporal_store(w[0], reinterpret_cast<std::uint32_t*>(dst));
+  }
+}
+
+template<int N>
+__device__ __forceinline__ void
+load_dwords(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[N])
+{
+  if constexpr (N % 4 == 0) {
+    const u32x4* q = reinterpret_cast<const u32x4*>(src);
+#pragma unroll
+    for (int c = 0; c < N / 4; ++c) {
+      const u32x4 d = __builtin_nontemporal_load(q + c);
+      w[4 * c] = d.x;
+      w[4 * c + 1] = d.y;
+      w[4 * c + 2] = d.z;
+      w[4 * c + 3] = d.w;
+    }
+  } else if constexpr (N == 2) {
+    typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 d = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(src));
+    w[0] = d.x;
+    w[1] = d.y;
+  } else {
+    w[0] = __builtin_nontemporal_load(reinterpret_cast<const std::uint32_t*>(src));
+  }
+}
+
+template<int U>
+__device__ __forceinline__ void
+store_exponent(std::uint8_t* __restrict__ out, std::uint64_t k, int e)
+{
+  using wt = typename word<U>::type;
+  reinterpret_cast<wt*>(out)[k] = static_cast<wt>(static_cast<std::int64_t>(e));
+}
+
+// max over the G lanes of an aligned lane group (G a power of two); for
+// G > 64 the wave maxima are combined through wred (one dword per wave).
+__device__ __forceinline__ std::uint32_t
+group_max(std::uint32_t v, int G, std::uint32_t* wred)
+{
+  const int gw = G < WAVE ? G : WAVE;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    if (off < gw) {
+      const std::uint32_t o = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off));
+      v = v > o ? v : o;
+    }
+  }
+  if (G > WAVE) {
+    const int wave = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+      wred[wave] = v;
+    }
+    __syncthreads();
+    const int wpg = G / WAVE;
+    const int w0 = wave & ~(wpg - 1);
+    v = wred[w0];
+    for (int i = 1; i < wpg; ++i) {
+      const std::uint32_t o = wred[w0 + i];
+      v = v > o ? v : o;
+    }
+    __syncthreads();
+  }
+  return v;
+}
+
+// OR a code into the LDS bit image at bit position pos
+template<int U>
+__device__ __forceinline__ void
+or_code(std::uint32_t* lds, std::uint32_t pos, std::uint64_t code, int bits)
+{
+  const std::uint32_t q = pos >> 5;
+  const int o = static_cast<int>(pos & 31u);
+  if constexpr (U <= 32) {
+    const std::uint64_t s = code << o;
+    atomicOr(lds + q, static_cast<std::uint32_t>(s));
+    if (o + bits > 32) {
+      atomicOr(lds + q + 1, static_cast<std::uint32_t>(s >> 32));
+    }
+  } else {
+    const std::uint64_t s = code << o;
+    const std::uint32_t lo = static_cast<std::uint32_t>(s);
+    if (lo) {
+      atomicOr(lds + q, lo);
+    }
+    if (o + bits > 32) {
+      atomicOr(lds + q + 1, static_cast<std::uint32_t>(s >> 32));
+    }
+    if (o + bits > 64) {
+      atomicOr(lds + q + 2, static_cast<std::uint32_t>(code >> (64 - o)));
+    }
+  }
+}
+
+template<int U>
+__device__ __forceinline__ std::uint64_t
+extract_code(const std::uint32_t* lds, std::uint32_t pos, int bits)
+{
+  const std::uint32_t q = pos >> 5;
+  const std::uint32_t o = pos & 31u;
+  const std::uint32_t lo = __builtin_amdgcn_alignbit(lds[q + 1], lds[q], o);
+  if constexpr (U <= 32) {
+    return bits == 32 ? lo : (lo & ((1u << bits) - 1u));
+  } else {
+    const std::uint32_t hi = __builtin_amdgcn_alignbit(lds[q + 2], lds[q + 1], o);
+    const std::uint64_t c = static_cast<std::uint64_t>(lo) | (static_cast<std::uint64_t>(hi) << 32);
+    return bits == 64 ? c : (c & ((std::uint64_t{ 1 } << bits) - 1));
+  }
+}
+
+// Per-value block exponents for one thread's VPT values.  Also stores the
+// exponents of the blocks that start at this thread's values.
+template<typename T, int U, int VPT>
+__device__ __forceinline__ void
+block_exponents(const std::uint32_t (&key)[VPT],
+                int (&E)[VPT],
+                std::uint64_t idx0,
+                const tplan& p,
+                std::uint8_t* __restrict__ out,
+                std::uint32_t* wred)
+{
+  const int B = 1 << p.lb;
+  if (B >= VPT) {
+    std::uint32_t m = key[0];
+#pragma unroll
+    for (int v = 1; v < VPT; ++v) {
+      m = m > key[v] ? m : key[v];
+    }
+    m = group_max(m, B / VPT, wred);
+    const int e = block_exponent_from_key<T, U>(m);
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      E[v] = e;
+    }
+    if ((idx0 & static_cast<std::uint64_t>(B - 1)) == 0) {
+      const std::uint64_t k = idx0 >> p.lb;
+      if (k < p.nb) {
+        store_exponent<U>(out, k, e);
+      }
+    }
+  } else {
+    // blocks of 1, 2 or 4 values inside the thread
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int w = (B == 1) ? 1 : (B == 2) ? 2 : 4;
+      if (s % w == 0) {
+        std::uint32_t m = key[s];
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+          if (i < w && s + i < VPT) {
+            m = m > key[s + i] ? m : key[s + i];
+          }
+        }
+        const int e = block_exponent_from_key<T, U>(m);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (i < w && s + i < VPT) {
+            E[s + i] = e;
+          }
+        }
+        const std::uint64_t k = (idx0 + s) >> p.lb;
+        if (k < p.nb) {
+          store_exponent<U>(out, k, e);
+        }
+      }
+    }
+  }
+}
+
+template<typename T, int U, int VPT, bool PACKED>
+__device__ __forceinline__ void
+compress_tile(const typename fmt<T>::uw* __restrict__ in,
+              std::uint8_t* __restrict__ out,
+              const tplan& p,
+              std::uint64_t tile,
+              std::uint32_t* lds,
+              std::uint32_t* wred)
+{
+  using uw = typename fmt<T>::uw;
+  constexpr int TILE = WG * VPT;
+  const int t = static_cast<int>(threadIdx.x);
+  const std::uint64_t base = tile * TILE;
+  const std::uint64_t idx0 = base + static_cast<std::uint64_t>(t) * VPT;
+
+  uw x[VPT];
+  if (idx0 + VPT <= p.n) {
+    load_values<T, VPT>(in + idx0, x);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      x[v] = (idx0 + v < p.n) ? in[idx0 + v] : uw{ 0 };
+    }
+  }
+  std::uint32_t key[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    key[v] = max_key<T>(x[v]);
+  }
+  int E[VPT];
+  block_exponents<T, U, VPT>(key, E, idx0, p, out, wred);
+
+  uw code[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    code[v] = encode_value<T>(x[v], E[v], p.bits);
+  }
+
+  std::uint8_t* pay = out + p.R;
+  if (!PACKED) {
+    // bits == U: payload word i holds value i; words [n, nb*B) get zero codes
+    const std::uint64_t limit = p.nb << p.lb;
+    if (idx0 + VPT <= limit) {
+      std::uint32_t w[VPT * U / 32 > 0 ? VPT * U / 32 : 1];
+      pack_words<U, VPT>(code, w);
+      store_dwords(pay + idx0 * (U / 8), w);
+    } else {
+      using wt = typename word<U>::type;
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) {
+        if (idx0 + v < limit) {
+          reinterpret_cast<wt*>(pay)[idx0 + v] = static_cast<wt>(code[v]);
+        }
+      }
+    }
+  } else {
+    const int B = 1 << p.lb;
+    const std::uint64_t k0 = base >> p.lb;
+    const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
+    const std::uint32_t tile_bytes = TB * p.CB;
+    u32x4* l4 = reinterpret_cast<u32x4*>(lds);
+    for (std::uint32_t c = t; c < tile_bytes / 16; c += WG) {
+      l4[c] = u32x4{ 0u, 0u, 0u, 0u };
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      if (code[v] != 0) {
+        const std::uint32_t local = static_cast<std::uint32_t>(t * VPT + v);
+        const std::uint32_t kl = local >> p.lb;
+        const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
+        const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
+        or_code<U>(lds, pos, static_cast<std::uint64_t>(code[v]), p.bits);
+      }
+    }
+    __syncthreads();
+    const std::uint64_t left = p.nb - k0;
+    const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
+    const std::uint32_t nbytes = blocks * p.CB;
+    std::uint8_t* dst = pay + k0 * p.CB;
+    u32x4* d4 = reinterpret_cast<u32x4*>(dst);
+    for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
+      __builtin_nontemporal_store(l4[c], d4 + c);
+    }
+    const std::uint32_t tail = nbytes & 15u;
+    if (static_cast<std::uint32_t>(t) < tail) {
+      const std::uint32_t o = nbytes - tail + t;
+      dst[o] = reinterpret_cast<const std::uint8_t*>(lds)[o];
+    }
+    __syncthreads();
+  }
+}
+
+template<typename T, int U, int VPT, bool PACKED>
+__device__ __forceinline__ void
+decompress_tile(const std::uint8_t* __restrict__ in,
+                typename fmt<T>::uw* __restrict__ out,
+                const tplan& p,
+                std::uint64_t tile,
+                std::uint32_t* lds)
+{
+  using uw = typename fmt<T>::uw;
+  constexpr int TILE = WG * VPT;
+  const int t = static_cast<int>(threadIdx.x);
+  const std::uint64_t base = tile * TILE;
+  const std::uint64_t idx0 = base + static_cast<std::uint64_t>(t) * VPT;
+  const int B = 1 << p.lb;
+  const std::uint8_t* pay = in + p.R;
+
+  if (PACKED) {
+    const std::uint64_t k0 = base >> p.lb;
+    const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
+    const std::uint64_t left = p.nb - k0;
+    const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
+    const std::uint32_t nbytes = blocks * p.CB;
+    const std::uint8_t* src = pay + k0 * p.CB;
+    const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
+    u32x4* l4 = reinterpret_cast<u32x4*>(lds);
+    for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
+      l4[c] = __builtin_nontemporal_load(s4 + c);
+    }
+    const std::uint32_t tail = nbytes & 15u;
+    if (static_cast<std::uint32_t>(t) < tail) {
+      const std::uint32_t o = nbytes - tail + t;
+      reinterpret_cast<std::uint8_t*>(lds)[o] = src[o];
+    }
+    __syncthreads();
+  }
+
+  int E[VPT];
+  if (B >= VPT) {
+    const int e = idx0 < p.n ? load_block_exponent<U>(in, idx0 >> p.lb) : 0;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      E[v] = e;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      E[v] = (idx0 + v < p.n) ? load_block_exponent<U>(in, (idx0 + v) >> p.lb) : 0;
+    }
+  }
+
+  uw code[VPT];
+  if (!PACKED) {
+    const std::uint64_t limit = p.nb << p.lb;
+    if (idx0 + VPT <= limit) {
+      std::uint32_t w[VPT * U / 32 > 0 ? VPT * U / 32 : 1];
+      load_dwords(pay + idx0 * (U / 8), w);
+      unpack_words<U, VPT>(w, code);
+    } else {
+      using wt = typename word<U>::type;
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) {
+        code[v] = (idx0 + v < limit) ? static_cast<uw>(reinterpret_cast<const wt*>(pay)[idx0 + v]) : uw{ 0 };
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      const std::uint32_t local = static_cast<std::uint32_t>(t * VPT + v);
+      const std::uint32_t kl = local >> p.lb;
+      const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
+      const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
+      code[v] = static_cast<uw>(extract_code<U>(lds, pos, p.bits));
+    }
+  }
+
+  uw y[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) {
+    y[v] = decode_value<T>(code[v], E[v], p.bits);
+  }
+  if (idx0 + VPT <= p.n) {
+    store_values<T, VPT>(out + idx0, y);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      if (idx0 + v < p.n) {
+        out[idx0 + v] = y[v];
+      }
+    }
+  }
+  if (PACKED) {
+    __syncthreads();
+  }
+}
+
+// Zero the exponent pad [nb*U/8, R) of a stream (the reference never writes
+// it; a canonical stream has zeros there).
+__device__ __forceinline__ void
+zero_pad(std::uint8_t* __restrict__ out, std::uint64_t nb, int U, std::uint64_t R, int t, int nthreads)
+{
+  const std::uint64_t x = nb * static_cast<std::uint64_t>(U / 8);
+  for (std::uint64_t i = x + t; i < R; i += nthreads) {
+    out[i] = 0;
+  }
+}
+
+// logical tile of workgroup `bid` such that each of the 8 XCDs (which receive
+// workgroups round-robin) processes one contiguous range of tiles
+__device__ __forceinline__ std::uint32_t
+xcd_tile(std::uint32_t bid, std::uint32_t ntiles)
+{
+  const std::uint32_t xcd = bid & 7u;
+  const std::uint32_t local = bid >> 3;
+  const std::uint32_t q = ntiles >> 3;
+  const std::uint32_t r = ntiles & 7u;
+  return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+} // namespace tile
+} // namespace frsz2

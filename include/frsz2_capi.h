@@ -1,0 +1,134 @@
+/*
+ * C ABI of the MI355X-native FRSZ2 codec (libfrsz2_hip.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory on
+ * the current device; `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  Every call is stream-ordered and asynchronous unless its
+ * comment says otherwise; the library keeps no global state besides a
+ * thread-local last-error string, so calls on different streams or threads
+ * may run concurrently (the reference plugin declares
+ * pressio:thread_safe = multiple, src/frsz2.cc:40).
+ *
+ * dtype: FRSZ2_FLOAT32 or FRSZ2_FLOAT64.
+ * bits:  4..32 for float32, 4..64 for float64 (src/frsz2.cc:86-101 lists the
+ *        same ranges; include/frsz2.hpp:511-517 asserts 4 <= bits <= W).
+ * block: the exponent block size B ("max_exp_block_size"), any value in
+ *        1..FRSZ2_MAX_BLOCK.  Powers of two up to 1024 take the tiled
+ *        kernels; other sizes take a slower one-thread-per-block kernel.
+ *
+ * Status codes follow the reference plugin: 0 ok, 1 unsupported dtype
+ * (src/frsz2.cc:164, :181), 2 unsupported bits / block size with the
+ * reference's message text "Unsupported number of bits: N" /
+ * "Unsupported exponent block size: N" (include/frsz2.hpp:1300, :1326).
+ * Codes 3..5 are new: capacity too small, bad pointer or alignment, HIP
+ * runtime error.  frsz2_last_error() returns the message of the calling
+ * thread's last failure.
+ */
+#ifndef FRSZ2_CAPI_H
+#define FRSZ2_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* frsz2_stream_t;
+
+enum
+{
+  FRSZ2_FLOAT32 = 0,
+  FRSZ2_FLOAT64 = 1
+};
+
+enum
+{
+  FRSZ2_OK = 0,
+  FRSZ2_EDTYPE = 1,
+  FRSZ2_EPARAM = 2,
+  FRSZ2_ECAPACITY = 3,
+  FRSZ2_EARG = 4,
+  FRSZ2_EHIP = 5
+};
+
+#define FRSZ2_MAX_BLOCK (1u << 24)
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* frsz2_last_error(void);
+
+/* Library version string. */
+const char* frsz2_version(void);
+
+/* 0 if (dtype, bits, block) is supported, else 1 or 2 with the message set.
+ * Replaces the supported-list checks of the runtime dispatch
+ * (include/frsz2.hpp:1296-1347, src/frsz2.cc:83-134). */
+int frsz2_check(int dtype, int bits, uint64_t block);
+
+/* Exact stream size in bytes for n values: exponent region rounded up to the
+ * next multiple of 256 (always 1..256 bytes of pad) plus ceil(n/B) blocks.
+ * Replaces frsz2_compressor::compute_compressed_memory_size_byte
+ * (include/frsz2.hpp:542-548).  Host-only, synchronous. */
+int frsz2_compressed_size(int dtype, int bits, uint64_t block, uint64_t n, uint64_t* bytes);
+
+/* Compress n values at d_in into the stream at d_out (capacity
+ * out_capacity >= frsz2_compressed_size).  Writes every byte of the stream:
+ * exponents, a zeroed pad, all payload words (the words of a partial last
+ * block that hold no code are written as zero).  d_in must be aligned to the
+ * value size and d_out to 8 bytes.
+ * Replaces frsz2_compressor::compress_cpu_impl (include/frsz2.hpp:1181-1193)
+ * and the compress_gpu kernel (include/frsz2.hpp:1258-1274). */
+int frsz2_compress(int dtype,
+                   int bits,
+                   uint64_t block,
+                   const void* d_in,
+                   uint64_t n,
+                   void* d_out,
+                   uint64_t out_capacity,
+                   frsz2_stream_t stream);
+
+/* Decompress the stream at d_in (in_bytes >= frsz2_compressed_size) into n
+ * values at d_out.
+ * Replaces frsz2_compressor::decompress_cpu_impl (include/frsz2.hpp:1242-1249)
+ * and the decompress_gpu kernel (include/frsz2.hpp:1276-1288). */
+int frsz2_decompress(int dtype,
+                     int bits,
+                     uint64_t block,
+                     const void* d_in,
+                     uint64_t in_bytes,
+                     uint64_t n,
+                     void* d_out,
+                     frsz2_stream_t stream);
+
+/* Random-access decode: d_out[i] = value d_idx[i] of the stream, for i < m.
+ * An index >= n yields 0 (the reference does not check; this reads nothing
+ * out of bounds instead).
+ * Replaces frsz2_compressor::decompress_cpu_element (include/frsz2.hpp:1199-1214)
+ * and decompress_gpu_element (include/frsz2.hpp:1096-1104). */
+int frsz2_gather(int dtype,
+                 int bits,
+                 uint64_t block,
+                 const void* d_in,
+                 uint64_t in_bytes,
+                 uint64_t n,
+                 const uint64_t* d_idx,
+                 uint64_t m,
+                 void* d_out,
+                 frsz2_stream_t stream);
+
+/* Host-memory variants: stage through device memory, run the HIP kernels and
+ * copy the result back; synchronous.  They keep the host-in / host-out
+ * meaning of the reference's CPU entry points and of the LibPressio plugin
+ * (src/frsz2.cc:152-186), which dereference host buffers. */
+int frsz2_compress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t n, void* h_out,
+                        uint64_t out_capacity);
+int frsz2_decompress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
+                          void* h_out);
+int frsz2_gather_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
+                      const uint64_t* h_idx, uint64_t m, void* h_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
