@@ -1,0 +1,201 @@
+"""FRSZ2 on one MI355X: compress + decompress throughput of the plain kernels.
+
+Default workload: a float64 Gaussian random field of 512^3 values (1 GiB,
+power spectrum k^-3, offset 1e3) at bits = 32, B = 32.  One step = one
+compress launch + one decompress launch over the resident array; inputs are
+in HBM before the timed region starts.  Throughput counts algorithmic bytes:
+compress reads n*8 and writes the stream, decompress the reverse.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--side 512]
+                    [--cpu-sample N] [--no-verify]
+
+Prints one JSON line.  ``roofline`` is for the dominant kernel (timed with
+HIP events on the stream the kernels run on); ``cpu_baseline`` times the
+reference CPU codec (oracle/_ref, built from the reference sources) on a
+bounded sample of the same field, single-threaded like the reference.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=512)
+    ap.add_argument("--bits", type=int, default=32)
+    ap.add_argument("--block", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 25, help="values in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(x_host, bits, block, sample):
+    """Time the reference CPU codec (or the C restatement when the reference
+    build is absent) on the first `sample` values; best of 2 passes."""
+    import numpy as np
+
+    from oracle.oracle import Port, Reference, compressed_bytes, reference_available
+
+    xs = np.ascontiguousarray(x_host[:sample])
+    kind = "reference" if reference_available() else "port"
+    codec = Reference() if kind == "reference" else Port()
+    tc, td = [], []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        s = codec.compress(xs, bits, block)
+        t1 = time.perf_counter()
+        codec.decompress(s, xs.size, xs.dtype, bits, block)
+        t2 = time.perf_counter()
+        tc.append(t1 - t0)
+        td.append(t2 - t1)
+    nbytes = xs.nbytes + compressed_bytes(bits, block, xs.size)
+    tcb, tdb = min(tc), min(td)
+    res = {
+        "value": round(2 * nbytes / (tcb + tdb) / 1e9, 4),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"first {xs.size} values ({xs.nbytes / 2**20:.0f} MiB) of the same field, compress+decompress, "
+                  f"best of 2, single thread",
+        "compress_gbs": round(nbytes / tcb / 1e9, 4),
+        "decompress_gbs": round(nbytes / tdb / 1e9, 4),
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+    # the block-parallel restatement on all host cores (same bytes, OpenMP)
+    port = Port().with_threads(0)
+    t0 = time.perf_counter()
+    s = port.compress(xs, bits, block)
+    t1 = time.perf_counter()
+    port.decompress(s, xs.size, xs.dtype, bits, block)
+    t2 = time.perf_counter()
+    res["port_all_cores"] = {"threads": port.max_threads(), "value": round(2 * nbytes / (t2 - t0) / 1e9, 4),
+                             "unit": "GB/s"}
+    return res
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (the codec "
+                                   "shards by exponent-block ranges, see the design notes)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import frsz_amd
+    from frsz_amd.workloads import gaussian_field_torch
+
+    dev = torch.device("cuda:0")
+    bits, block = args.bits, args.block
+    side = args.side
+    print(f"bench: generating float64 field {side}^3", file=sys.stderr, flush=True)
+    x = gaussian_field_torch(side, seed=2, device=dev).reshape(-1).contiguous()
+    n = x.numel()
+    size = frsz_amd.compressed_size(torch.float64, bits, block, n)
+    stream = torch.empty(size, dtype=torch.uint8, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        frsz_amd.compress_into(x, stream, bits, block)
+        if ev is not None:
+            ev[1].record()
+        frsz_amd.decompress_into(stream, y, bits, block)
+        if ev is not None:
+            ev[2].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tc = [e[0].elapsed_time(e[1]) * 1e-3 for e in evs]
+    td = [e[1].elapsed_time(e[2]) * 1e-3 for e in evs]
+    alg = n * 8 + size  # algorithmic bytes per launch, each direction
+    comp_gbs = alg / (sum(tc) / len(tc)) / 1e9
+    decomp_gbs = alg / (sum(td) / len(td)) / 1e9
+    ms_per_step = wall / args.steps * 1e3
+    value = 2 * alg / (wall / args.steps) / 1e9
+
+    verified = None
+    x_host = None
+    if not args.no_verify or not args.no_cpu:
+        x_host = x.cpu().numpy()
+    if not args.no_verify:
+        from oracle.oracle import Port
+
+        port = Port().with_threads(0)
+        want = port.compress(x_host, bits, block)
+        ok_s = np.array_equal(stream.cpu().numpy(), want)
+        ok_y = np.array_equal(y.cpu().numpy().view(np.uint64),
+                              port.decompress(want, n, np.float64, bits, block).view(np.uint64))
+        verified = bool(ok_s and ok_y)
+
+    dom = "compress" if sum(tc) >= sum(td) else "decompress"
+    dom_t = (sum(tc) if dom == "compress" else sum(td)) / args.steps
+    achieved = alg / dom_t / 1e9
+    line = {
+        "metric": "frsz2 comp/decomp GB/s + %HBM peak, batched vs plain (bit-exact); serve p99 ms",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Gaussian random field k^-3 + 1e3, seeded, generated on the GPU)",
+        "config": {"workload": f"plain single array: float64 {side}^3 ({n * 8 / 2**30:.2f} GiB), bits={bits}, "
+                               f"B={block}, one compress + one decompress launch per step",
+                   "n": n, "bits": bits, "block": block, "stream_bytes": size},
+        "compress_gbs": round(comp_gbs, 2),
+        "decompress_gbs": round(decomp_gbs, 2),
+        "compress_pct_hbm": round(100 * comp_gbs / HBM_PEAK_GBS, 2),
+        "decompress_pct_hbm": round(100 * decomp_gbs / HBM_PEAK_GBS, 2),
+        "compress_us": round(sum(tc) / len(tc) * 1e6, 2),
+        "decompress_us": round(sum(td) / len(td) * 1e6, 2),
+        "bit_exact": verified,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if not args.no_cpu:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(x_host, bits, block, min(args.cpu_sample, n))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

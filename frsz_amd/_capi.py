@@ -29,6 +29,12 @@ class Frsz2Error(RuntimeError):
 
 
 def _load() -> ctypes.CDLL:
+    # torch ships its own libamdhip64 (SONAME libamdhip64.so.7).  Loading it
+    # first makes the dynamic linker bind this library's libamdhip64.so.7 to
+    # that copy; loading ours first would leave two HIP runtimes in the
+    # process, and the second one finds no device.
+    import torch  # noqa: F401
+
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: the frsz2 HIP library has not been built "

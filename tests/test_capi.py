@@ -1,0 +1,95 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every function
+include/*.h declares, and rejects bad parameters with the reference's codes
+and messages before touching the GPU (no kernel is launched here)."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+capi = pytest.importorskip("frsz_amd._capi")
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(frsz2_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(capi.lib, n)]
+    assert not missing, missing
+
+
+def test_version():
+    assert "frsz2" in capi.version()
+
+
+@pytest.mark.parametrize("dtype,bits,block,rc,msg", [
+    (2, 16, 32, 1, "unsupported dtype"),
+    (0, 3, 32, 2, "Unsupported number of bits: 3"),
+    (0, 33, 32, 2, "Unsupported number of bits: 33"),
+    (1, 65, 32, 2, "Unsupported number of bits: 65"),
+    (1, 16, 0, 2, "Unsupported exponent block size: 0"),
+    (0, 16, (1 << 24) + 1, 2, f"Unsupported exponent block size: {(1 << 24) + 1}"),
+])
+def test_check_rejects(dtype, bits, block, rc, msg):
+    assert capi.lib.frsz2_check(dtype, bits, block) == rc
+    assert capi.last_error() == msg
+    # the device entry points validate before any launch
+    assert capi.lib.frsz2_compress(dtype, bits, block, None, 10, None, 0, None) == rc
+    assert capi.last_error() == msg
+
+
+def test_messages_match_reference_dispatch():
+    # the reference's runtime dispatch messages (include/frsz2.hpp:1300, :1326)
+    for e in json.load(open(os.path.join(GOLDEN, "dispatch_errors.json"))):
+        bits, block = e["bits"], e["block"]
+        if bits == 5 or block in (3, 2048):
+            continue  # supported here: any bits 4..W and any block size 1..2^24
+        assert capi.lib.frsz2_check(0, bits, block) == e["rc"]
+        assert capi.last_error() == e["message"]
+
+
+@pytest.mark.parametrize("dtype,bits,B,n", [(0, 16, 32, 1 << 20), (1, 32, 32, 512 ** 3), (0, 21, 16, 100_000_037),
+                                            (1, 9, 5, 2049), (0, 4, 1, 0)])
+def test_compressed_size_matches_layout(dtype, bits, B, n):
+    from oracle.oracle import compressed_bytes
+
+    out = ctypes.c_uint64()
+    assert capi.lib.frsz2_compressed_size(dtype, bits, B, n, ctypes.byref(out)) == 0
+    assert out.value == compressed_bytes(bits, B, n)
+
+
+def test_capacity_and_pointer_errors():
+    fake = ctypes.c_void_p(1 << 20)
+    # capacity below the stream size
+    assert capi.lib.frsz2_compress(0, 16, 32, fake, 1000, fake, 100, None) == capi.ECAPACITY
+    assert capi.lib.frsz2_decompress(0, 16, 32, fake, 100, 1000, fake, None) == capi.ECAPACITY
+    # misaligned value pointer
+    bad = ctypes.c_void_p((1 << 20) + 2)
+    assert capi.lib.frsz2_compress(0, 16, 32, bad, 1000, fake, 1 << 20, None) == capi.EARG
+    assert "misaligned" in capi.last_error()
+    assert capi.lib.frsz2_compress(0, 16, 32, fake, 1000, None, 1 << 20, None) == capi.EARG
+
+
+def test_python_plugin_surface_defaults():
+    from frsz_amd import Frsz2Compressor
+
+    c = Frsz2Compressor()
+    assert c.get_options() == {"frsz2:bits": 32, "frsz2:max_exp_block_size": 32}
+    c.set_options({"frsz2:bits": 16})
+    assert c.clone().get_options()["frsz2:bits"] == 16
+    assert c.get_configuration()["pressio:thread_safe"] == "multiple"

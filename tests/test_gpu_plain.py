@@ -1,0 +1,206 @@
+"""Parity of the plain HIP kernels (compress / decompress / gather through the
+C ABI) with the golden vectors of the reference codec and with the oracle.
+
+Bar: bit-exact stream bytes, decoded bit patterns and gathered values."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, GOLDEN_CASES
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def fz():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frsz_amd
+
+    return frsz_amd
+
+
+def tdt(dtype):
+    return torch.float32 if np.dtype(dtype) == np.float32 else torch.float64
+
+
+def ubits(a):
+    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+def gpu_compress(fz, x, bits, B):
+    xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    s = fz.compress(xd, bits, B)
+    return s
+
+
+def gpu_roundtrip(fz, x, bits, B):
+    s = gpu_compress(fz, x, bits, B)
+    y = fz.decompress(s, x.size, tdt(x.dtype), bits, B)
+    torch.cuda.synchronize()
+    return s.cpu().numpy(), y.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES, ids=repr)
+def test_golden_vectors(fz, case):
+    print(f"golden {case!r}", flush=True)
+    s, y = gpu_roundtrip(fz, case.x, case.bits, case.B)
+    assert np.array_equal(s, case.stream), "stream bytes differ from the reference"
+    assert np.array_equal(ubits(y), ubits(case.y)), "decoded values differ from the reference"
+    if case.gidx.size:
+        st = torch.from_numpy(case.stream).cuda()
+        idx = torch.from_numpy(case.gidx.astype(np.int64)).cuda()
+        g = fz.gather(st, case.n, tdt(case.dtype), case.bits, case.B, idx).cpu().numpy()
+        assert np.array_equal(ubits(g), ubits(case.gval))
+
+
+def test_plain_roundtrip_2p20_float32(fz, port):
+    """2^20 smooth-signal float32 values at bits 16, B 32: golden hashes."""
+    from frsz_amd.workloads import smooth_signal
+
+    meta = json.load(open(os.path.join(GOLDEN, "plain_roundtrip_f32.json")))
+    x = smooth_signal(meta["n"], seed=1)
+    s, y = gpu_roundtrip(fz, x, meta["bits"], meta["block"])
+    assert s.size == meta["stream_bytes"]
+    if hashlib.sha256(x.tobytes()).hexdigest() == meta["sha256_input"]:
+        assert hashlib.sha256(s.tobytes()).hexdigest() == meta["sha256_stream"]
+        assert hashlib.sha256(y.tobytes()).hexdigest() == meta["sha256_decoded"]
+    assert np.array_equal(s, port.compress(x, meta["bits"], meta["block"]))
+    assert np.array_equal(ubits(y), ubits(port.decompress(s, x.size, np.float32, meta["bits"], meta["block"])))
+
+
+def mixed_values(rng, n, dtype):
+    W = 32 if dtype == np.float32 else 64
+    span = 30 if W == 32 else 300
+    x = (np.exp2(rng.uniform(-span, span, n)) * rng.standard_normal(n)).astype(dtype)
+    x[rng.random(n) < 0.05] = 0
+    if n > 200:
+        s = int(rng.integers(0, n - 150))
+        x[s:s + 150] = 0
+    sub = rng.random(n) < 0.01
+    if W == 32:
+        x[sub] = rng.integers(1, 1 << 23, int(sub.sum()), dtype=np.uint32).view(np.float32)
+    else:
+        x[sub] = rng.integers(1, 1 << 52, int(sub.sum()), dtype=np.uint64).view(np.float64)
+    return x
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 3, 5, 9, 100, 2048])
+def test_random_parity_all_bits(fz, port, dtype, B):
+    """Every bit width 4..W against the oracle, sizes spanning several tiles
+    plus a ragged tail; power-of-two B <= 1024 take the tiled kernels, the
+    other sizes the one-thread-per-block kernel."""
+    rng = np.random.default_rng(B * 7 + (dtype == np.float64))
+    W = 32 if dtype == np.float32 else 64
+    for bits in range(4, W + 1):
+        if W == 64 and bits <= 8:
+            continue
+        n = int(rng.integers(1, 9000))
+        x = mixed_values(rng, n, dtype)
+        print(f"{np.dtype(dtype).name} bits={bits} B={B} n={n}", flush=True)
+        s, y = gpu_roundtrip(fz, x, bits, B)
+        want = port.compress(x, bits, B)
+        assert np.array_equal(s, want), (bits, B, n)
+        assert np.array_equal(ubits(y), ubits(port.decompress(want, n, dtype, bits, B))), (bits, B, n)
+
+
+@pytest.mark.parametrize("bits,B", [(8, 16), (8, 64), (7, 32), (5, 1), (4, 1024)])
+def test_double_low_bits_defined_inputs(fz, port, bits, B):
+    """float64 with bits <= 8 stores int8 block exponents; inputs whose
+    exponents fit int8 (and all-zero blocks) are well defined."""
+    rng = np.random.default_rng(bits + B)
+    n = 5000
+    x = np.exp2(rng.uniform(-100, 100, n)) * rng.standard_normal(n)
+    x[:2 * B] = 0
+    s, y = gpu_roundtrip(fz, x, bits, B)
+    want = port.compress(x, bits, B)
+    assert np.array_equal(s, want)
+    assert np.array_equal(ubits(y), ubits(port.decompress(want, n, np.float64, bits, B)))
+
+
+@pytest.mark.parametrize("n", [0, 1, 31, 32, 33, 1023, 1024, 1025, 4097, 262143])
+@pytest.mark.parametrize("bits,B", [(16, 32), (12, 16), (21, 64), (32, 1)])
+def test_ragged_sizes(fz, port, n, bits, B):
+    rng = np.random.default_rng(n)
+    x = mixed_values(rng, n, np.float32)
+    s, y = gpu_roundtrip(fz, x, bits, B)
+    assert np.array_equal(s, port.compress(x, bits, B))
+    assert np.array_equal(ubits(y), ubits(x if bits == 32 and B == 1 else port.decompress(s, n, np.float32, bits, B)))
+
+
+def test_no_write_outside_stream(fz):
+    """The codec writes exactly the stream's bytes: a guard band after it
+    keeps its fill (the reference GPU test's 0xFF + 128 B guard,
+    test/test_frsz_gpu.cu:331-359)."""
+    rng = np.random.default_rng(3)
+    for bits, B, n in ((16, 32, 5000), (21, 16, 4099), (9, 5, 2049), (32, 32, 1 << 16)):
+        x = torch.from_numpy(mixed_values(rng, n, np.float32)).cuda()
+        size = fz.compressed_size(torch.float32, bits, B, n)
+        buf = torch.full((size + 4096,), 0xAB, dtype=torch.uint8, device="cuda")
+        fz.compress_into(x, buf, bits, B)
+        tail = buf[size:].cpu().numpy()
+        assert np.all(tail == 0xAB)
+
+
+def test_unaligned_pointers_take_generic_path(fz, port):
+    rng = np.random.default_rng(4)
+    n, bits, B = 10001, 16, 32
+    x = mixed_values(rng, n + 1, np.float32)
+    xd = torch.from_numpy(x).cuda()[1:]  # 4-byte offset: not 16-byte aligned
+    s = fz.compress(xd, bits, B)
+    want = port.compress(x[1:], bits, B)
+    assert np.array_equal(s.cpu().numpy(), want)
+    out = torch.empty(n + 1, dtype=torch.float32, device="cuda")[1:]
+    fz.decompress_into(s, out, bits, B)
+    assert np.array_equal(ubits(out.cpu().numpy()), ubits(port.decompress(want, n, np.float32, bits, B)))
+
+
+def test_gather_out_of_range_is_zero(fz):
+    x = torch.ones(1000, dtype=torch.float32, device="cuda")
+    s = fz.compress(x, 16, 32)
+    idx = torch.tensor([0, 999, 1000, 1 << 40], dtype=torch.int64, device="cuda")
+    g = fz.gather(s, 1000, torch.float32, 16, 32, idx).cpu().numpy()
+    assert g[0] == 1.0 and g[1] == 1.0 and g[2] == 0.0 and g[3] == 0.0
+
+
+def test_deterministic_and_stream_ordered(fz):
+    from frsz_amd.workloads import smooth_signal_torch
+
+    x = smooth_signal_torch(1 << 22, seed=3, dtype=torch.float64)
+    s1 = fz.compress(x, 32, 32)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        side.wait_stream(torch.cuda.current_stream())
+        s2 = fz.compress(x, 32, 32)
+    torch.cuda.current_stream().wait_stream(side)
+    assert torch.equal(s1, s2)
+
+
+def test_host_memory_entry_points(fz, port):
+    """frsz2_*_host: host in, host out, same bytes as the device path."""
+    import ctypes
+
+    from frsz_amd import _capi
+
+    rng = np.random.default_rng(8)
+    x = mixed_values(rng, 7777, np.float64)
+    bits, B = 21, 16
+    size = port.compress(x, bits, B).size
+    out = np.zeros(size, np.uint8)
+    _capi.check(_capi.lib.frsz2_compress_host(1, bits, B, x.ctypes.data, x.size, out.ctypes.data, size))
+    assert np.array_equal(out, port.compress(x, bits, B))
+    y = np.empty_like(x)
+    _capi.check(_capi.lib.frsz2_decompress_host(1, bits, B, out.ctypes.data, size, x.size, y.ctypes.data))
+    assert np.array_equal(ubits(y), ubits(port.decompress(out, x.size, np.float64, bits, B)))
+    idx = rng.integers(0, x.size, 300).astype(np.uint64)
+    g = np.empty(300, np.float64)
+    _capi.check(_capi.lib.frsz2_gather_host(1, bits, B, out.ctypes.data, size, x.size, idx.ctypes.data, 300,
+                                            g.ctypes.data))
+    assert np.array_equal(ubits(g), ubits(y[idx.astype(np.int64)]))
+    del ctypes
