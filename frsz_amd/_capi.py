@@ -63,6 +63,10 @@ lib.frsz2_decompress.restype = _int
 lib.frsz2_decompress.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp, _vp]
 lib.frsz2_gather.restype = _int
 lib.frsz2_gather.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp, _u64, _vp, _vp]
+lib.frsz2_tune_set.restype = _int
+lib.frsz2_tune_set.argtypes = [_int, _int, _int, _int, _int]
+lib.frsz2_tune_get.restype = _int
+lib.frsz2_tune_get.argtypes = [_int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int)]
 lib.frsz2_compress_host.restype = _int
 lib.frsz2_compress_host.argtypes = [_int, _int, _u64, _vp, _u64, _vp, _u64]
 lib.frsz2_decompress_host.restype = _int
@@ -82,3 +86,13 @@ def last_error() -> str:
 
 def version() -> str:
     return lib.frsz2_version().decode()
+
+
+def tune_set(op: int, dtype: int, bits: int, values_per_thread: int, nontemporal: bool) -> None:
+    check(lib.frsz2_tune_set(op, dtype, bits, values_per_thread, int(bool(nontemporal))))
+
+
+def tune_get(op: int, dtype: int, bits: int):
+    v, nt = ctypes.c_int(), ctypes.c_int()
+    check(lib.frsz2_tune_get(op, dtype, bits, ctypes.byref(v), ctypes.byref(nt)))
+    return v.value, bool(nt.value)

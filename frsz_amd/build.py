@@ -1,7 +1,8 @@
 """In-tree build of the native parts (hipcc for gfx950, gcc / g++ for the
 host-only pieces).
 
-    python -m frsz_amd.build          # everything below
+    python frsz_amd/build.py          # everything below (run as a script: importing
+                                      # the package would load the old library)
     build_library()                   # frsz_amd/lib/libfrsz2_hip.so
     build_test_helpers()              # tests/native/*.so and the C++ API test
     build_oracle()                    # oracle/libfrsz2_oracle.so (+ oracle/_ref)

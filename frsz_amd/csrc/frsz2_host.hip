@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 
@@ -18,6 +19,50 @@ namespace frsz2 {
 
 namespace {
 thread_local std::string g_error;
+
+// tuning table of the tiled kernels: [op][dtype][U = 8,16,32,64][bits < U],
+// each entry vpt * 2 + nt.  Defaults come from measurements on MI355X;
+// frsz2_tune_set (the autotuner) overrides them per process.
+std::atomic<int> g_variant[2][2][4][2];
+std::atomic<bool> g_variant_init{ false };
+
+int
+u_index(int bits)
+{
+  const int U = storage_bits(bits);
+  return U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3;
+}
+
+void
+init_variants()
+{
+  if (g_variant_init.load(std::memory_order_acquire)) {
+    return;
+  }
+  for (int op = 0; op < 2; ++op)
+    for (int d = 0; d < 2; ++d)
+      for (int u = 0; u < 4; ++u)
+        for (int pk = 0; pk < 2; ++pk)
+          g_variant[op][d][u][pk].store(4 * 2 + 1, std::memory_order_relaxed);
+  g_variant_init.store(true, std::memory_order_release);
+}
+} // namespace
+
+variant
+get_variant(int op, int dtype, int bits)
+{
+  init_variants();
+  const int v = g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].load(std::memory_order_relaxed);
+  return variant{ v / 2, v % 2 };
+}
+
+int
+set_variant(int op, int dtype, int bits, int vpt, int nt)
+{
+  init_variants();
+  g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].store(vpt * 2 + (nt ? 1 : 0),
+                                                                             std::memory_order_relaxed);
+  return 0;
 }
 
 int
@@ -72,6 +117,40 @@ const char*
 frsz2_last_error(void)
 {
   return frsz2::g_error.c_str();
+}
+
+int
+frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int nontemporal)
+{
+  using namespace frsz2;
+  clear_error();
+  const int rc = check_params(dtype, bits, 1);
+  if (rc) {
+    return rc;
+  }
+  if ((op != OP_COMPRESS && op != OP_DECOMPRESS) ||
+      (values_per_thread != 4 && values_per_thread != 8 && values_per_thread != 16)) {
+    return set_error(FRSZ2_EARG, "op must be 0/1 and values_per_thread 4, 8 or 16");
+  }
+  return set_variant(op, dtype, bits, values_per_thread, nontemporal);
+}
+
+int
+frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* nontemporal)
+{
+  using namespace frsz2;
+  clear_error();
+  const int rc = check_params(dtype, bits, 1);
+  if (rc) {
+    return rc;
+  }
+  if ((op != OP_COMPRESS && op != OP_DECOMPRESS) || !values_per_thread || !nontemporal) {
+    return set_error(FRSZ2_EARG, "bad op or null pointer");
+  }
+  const variant v = get_variant(op, dtype, bits);
+  *values_per_thread = v.vpt;
+  *nontemporal = v.nt;
+  return FRSZ2_OK;
 }
 
 int

@@ -13,6 +13,21 @@ int set_error(int code, const std::string& msg);
 void clear_error();
 
 int check_params(int dtype, int bits, std::uint64_t block);
+
+// kernel variant of the tiled plain kernels: values per thread (4, 8, 16)
+// and whether global accesses carry the non-temporal hint
+enum
+{
+  OP_COMPRESS = 0,
+  OP_DECOMPRESS = 1
+};
+struct variant
+{
+  int vpt;
+  int nt;
+};
+variant get_variant(int op, int dtype, int bits);
+int set_variant(int op, int dtype, int bits, int vpt, int nt);
 int hip_status(hipError_t e, const char* what);
 
 int compress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t n, void* d_out,

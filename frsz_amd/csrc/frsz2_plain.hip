@@ -29,7 +29,7 @@ using tile::WG;
 
 // ---------------------------------------------------------------- kernels
 
-template<typename T, int U, int VPT, bool PACKED>
+template<typename T, int U, int VPT, bool PACKED, bool NT>
 __global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw* __restrict__ in,
                                                       std::uint8_t* __restrict__ out,
                                                       tplan p,
@@ -38,13 +38,13 @@ __global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw*
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
   const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
-  tile::compress_tile<T, U, VPT, PACKED>(in, out, p, t, lds, wred);
+  tile::compress_tile<T, U, VPT, PACKED, NT>(in, out, p, t, lds, wred);
   if (t == 0) {
     tile::zero_pad(out, p.nb, U, p.R, static_cast<int>(threadIdx.x), WG);
   }
 }
 
-template<typename T, int U, int VPT, bool PACKED>
+template<typename T, int U, int VPT, bool PACKED, bool NT>
 __global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __restrict__ in,
                                                         typename fmt<T>::uw* __restrict__ out,
                                                         tplan p,
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __re
 {
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
   const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
-  tile::decompress_tile<T, U, VPT, PACKED>(in, out, p, t, lds);
+  tile::decompress_tile<T, U, VPT, PACKED, NT>(in, out, p, t, lds);
 }
 
 // One thread per exponent block, any B; builds each payload word from the
@@ -150,7 +150,7 @@ grid_for(std::uint64_t work)
   return static_cast<unsigned>((work + WG - 1) / WG);
 }
 
-template<typename T, int U, int VPT>
+template<typename T, int U, int VPT, bool NT>
 hipError_t
 launch_compress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
 {
@@ -163,14 +163,16 @@ launch_compress_tiled(const void* in, void* out, const layout& L, int lb, hipStr
   const auto* x = static_cast<const typename fmt<T>::uw*>(in);
   auto* o = static_cast<std::uint8_t*>(out);
   if (L.bits == U) {
-    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+                       static_cast<std::uint32_t>(nt));
   } else {
-    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+                       static_cast<std::uint32_t>(nt));
   }
   return hipGetLastError();
 }
 
-template<typename T, int U, int VPT>
+template<typename T, int U, int VPT, bool NT>
 hipError_t
 launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
 {
@@ -183,9 +185,11 @@ launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipS
   const auto* x = static_cast<const std::uint8_t*>(in);
   auto* o = static_cast<typename fmt<T>::uw*>(out);
   if (L.bits == U) {
-    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+                       static_cast<std::uint32_t>(nt));
   } else {
-    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true>), dim3(nt), dim3(WG), 0, s, x, o, p, static_cast<std::uint32_t>(nt));
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+                       static_cast<std::uint32_t>(nt));
   }
   return hipGetLastError();
 }
@@ -224,20 +228,29 @@ launch_gather(const void* in, const layout& L, const std::uint64_t* idx, std::ui
   return hipGetLastError();
 }
 
-// values per thread of the tiled kernels, per (dtype, U)
-template<typename T, int U>
-struct vpt_of
-{
-  static constexpr int value = 4;
-};
-
+// Tiled launch with the (values per thread, non-temporal) variant of the
+// tuning table; B must not exceed the tile of that variant, else the
+// one-thread-per-block kernel runs.
 template<typename T, int U>
 hipError_t
 run_compress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
 {
-  constexpr int VPT = vpt_of<T, U>::value;
-  if (tiled && (static_cast<std::uint64_t>(WG) * VPT) >= L.B) {
-    return launch_compress_tiled<T, U, VPT>(in, out, L, lb, s);
+  const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
+  if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
+    switch (v.vpt * 2 + v.nt) {
+      case 8:
+        return launch_compress_tiled<T, U, 4, false>(in, out, L, lb, s);
+      case 9:
+        return launch_compress_tiled<T, U, 4, true>(in, out, L, lb, s);
+      case 16:
+        return launch_compress_tiled<T, U, 8, false>(in, out, L, lb, s);
+      case 17:
+        return launch_compress_tiled<T, U, 8, true>(in, out, L, lb, s);
+      case 32:
+        return launch_compress_tiled<T, U, 16, false>(in, out, L, lb, s);
+      default:
+        return launch_compress_tiled<T, U, 16, true>(in, out, L, lb, s);
+    }
   }
   return launch_compress_generic<T, U>(in, out, L, s);
 }
@@ -246,9 +259,22 @@ template<typename T, int U>
 hipError_t
 run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
 {
-  constexpr int VPT = vpt_of<T, U>::value;
-  if (tiled && (static_cast<std::uint64_t>(WG) * VPT) >= L.B) {
-    return launch_decompress_tiled<T, U, VPT>(in, out, L, lb, s);
+  const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
+  if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
+    switch (v.vpt * 2 + v.nt) {
+      case 8:
+        return launch_decompress_tiled<T, U, 4, false>(in, out, L, lb, s);
+      case 9:
+        return launch_decompress_tiled<T, U, 4, true>(in, out, L, lb, s);
+      case 16:
+        return launch_decompress_tiled<T, U, 8, false>(in, out, L, lb, s);
+      case 17:
+        return launch_decompress_tiled<T, U, 8, true>(in, out, L, lb, s);
+      case 32:
+        return launch_decompress_tiled<T, U, 16, false>(in, out, L, lb, s);
+      default:
+        return launch_decompress_tiled<T, U, 16, true>(in, out, L, lb, s);
+    }
   }
   return launch_decompress_generic<T, U>(in, out, L, s);
 }

@@ -54,7 +54,30 @@ constexpr int lds_dwords()
   return WG * VPT * 2 + 4;
 }
 
-template<typename T, int VPT>
+// 128/64/32-bit global access, optionally with the non-temporal hint
+template<bool NT, typename V>
+__device__ __forceinline__ V
+ld(const V* p)
+{
+  if constexpr (NT) {
+    return __builtin_nontemporal_load(p);
+  } else {
+    return *p;
+  }
+}
+
+template<bool NT, typename V>
+__device__ __forceinline__ void
+st(V v, V* p)
+{
+  if constexpr (NT) {
+    __builtin_nontemporal_store(v, p);
+  } else {
+    *p = v;
+  }
+}
+
+template<typename T, int VPT, bool NT>
 __device__ __forceinline__ void
 load_values(const typename fmt<T>::uw* __restrict__ p, typename fmt<T>::uw (&x)[VPT])
 {
@@ -62,7 +85,7 @@ load_values(const typename fmt<T>::uw* __restrict__ p, typename fmt<T>::uw (&x)[
   const u32x4* q = reinterpret_cast<const u32x4*>(p);
 #pragma unroll
   for (int c = 0; c < NV; ++c) {
-    const u32x4 d = __builtin_nontemporal_load(q + c);
+    const u32x4 d = ld<NT>(q + c);
     if (sizeof(T) == 4) {
       x[4 * c + 0] = d.x;
       x[4 * c + 1] = d.y;
@@ -75,7 +98,7 @@ load_values(const typename fmt<T>::uw* __restrict__ p, typename fmt<T>::uw (&x)[
   }
 }
 
-template<typename T, int VPT>
+template<typename T, int VPT, bool NT>
 __device__ __forceinline__ void
 store_values(typename fmt<T>::uw* __restrict__ p, const typename fmt<T>::uw (&y)[VPT])
 {
@@ -95,7 +118,7 @@ store_values(typename fmt<T>::uw* __restrict__ p, const typename fmt<T>::uw (&y)
       d.z = static_cast<std::uint32_t>(y[2 * c + 1]);
       d.w = static_cast<std::uint32_t>(static_cast<std::uint64_t>(y[2 * c + 1]) >> 32);
     }
-    __builtin_nontemporal_store(d, q + c);
+    st<NT>(d, q + c);
   }
 }
 
@@ -146,7 +169,7 @@ unpack_words(const std::uint32_t (&w)[VPT * U / 32 > 0 ? VPT * U / 32 : 1], UW (
   }
 }
 
-template<int N>
+template<bool NT, int N>
 __device__ __forceinline__ void
 store_dwords(std::uint8_t* __restrict__ dst, const std::uint32_t (&w)[N])
 {
@@ -159,20 +182,20 @@ store_dwords(std::uint8_t* __restrict__ dst, const std::uint32_t (&w)[N])
       d.y = w[4 * c + 1];
       d.z = w[4 * c + 2];
       d.w = w[4 * c + 3];
-      __builtin_nontemporal_store(d, q + c);
+      st<NT>(d, q + c);
     }
   } else if constexpr (N == 2) {
     typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
     u32x2 d;
     d.x = w[0];
     d.y = w[1];
-    __builtin_nontemporal_store(d, reinterpret_cast<u32x2*>(dst));
+    st<NT>(d, reinterpret_cast<u32x2*>(dst));
   } else {
-    __builtin_nontemporal_store(w[0], reinterpret_cast<std::uint32_t*>(dst));
+    st<NT>(w[0], reinterpret_cast<std::uint32_t*>(dst));
   }
 }
 
-template<int N>
+template<bool NT, int N>
 __device__ __forceinline__ void
 load_dwords(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[N])
 {
@@ -180,7 +203,7 @@ load_dwords(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[N])
     const u32x4* q = reinterpret_cast<const u32x4*>(src);
 #pragma unroll
     for (int c = 0; c < N / 4; ++c) {
-      const u32x4 d = __builtin_nontemporal_load(q + c);
+      const u32x4 d = ld<NT>(q + c);
       w[4 * c] = d.x;
       w[4 * c + 1] = d.y;
       w[4 * c + 2] = d.z;
@@ -188,11 +211,11 @@ load_dwords(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[N])
     }
   } else if constexpr (N == 2) {
     typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 d = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(src));
+    const u32x2 d = ld<NT>(reinterpret_cast<const u32x2*>(src));
     w[0] = d.x;
     w[1] = d.y;
   } else {
-    w[0] = __builtin_nontemporal_load(reinterpret_cast<const std::uint32_t*>(src));
+    w[0] = ld<NT>(reinterpret_cast<const std::uint32_t*>(src));
   }
 }
 
@@ -310,35 +333,40 @@ block_exponents(const std::uint32_t (&key)[VPT],
       }
     }
   } else {
-    // blocks of 1, 2 or 4 values inside the thread
+    // B < VPT: whole blocks inside the thread; butterfly max over runs of B
+    std::uint32_t m[VPT];
 #pragma unroll
-    for (int s = 0; s < VPT; ++s) {
-      const int w = (B == 1) ? 1 : (B == 2) ? 2 : 4;
-      if (s % w == 0) {
-        std::uint32_t m = key[s];
+    for (int v = 0; v < VPT; ++v) {
+      m[v] = key[v];
+    }
 #pragma unroll
-        for (int i = 1; i < 4; ++i) {
-          if (i < w && s + i < VPT) {
-            m = m > key[s + i] ? m : key[s + i];
-          }
+    for (int s = 1; s < VPT; s <<= 1) {
+      if (s < B) {
+        std::uint32_t t[VPT];
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) {
+          t[v] = m[v] > m[v ^ s] ? m[v] : m[v ^ s];
         }
-        const int e = block_exponent_from_key<T, U>(m);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (i < w && s + i < VPT) {
-            E[s + i] = e;
-          }
+        for (int v = 0; v < VPT; ++v) {
+          m[v] = t[v];
         }
-        const std::uint64_t k = (idx0 + s) >> p.lb;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      E[v] = block_exponent_from_key<T, U>(m[v]);
+      if ((v & (B - 1)) == 0) {
+        const std::uint64_t k = (idx0 + v) >> p.lb;
         if (k < p.nb) {
-          store_exponent<U>(out, k, e);
+          store_exponent<U>(out, k, E[v]);
         }
       }
     }
   }
 }
 
-template<typename T, int U, int VPT, bool PACKED>
+template<typename T, int U, int VPT, bool PACKED, bool NT>
 __device__ __forceinline__ void
 compress_tile(const typename fmt<T>::uw* __restrict__ in,
               std::uint8_t* __restrict__ out,
@@ -355,7 +383,7 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
 
   uw x[VPT];
   if (idx0 + VPT <= p.n) {
-    load_values<T, VPT>(in + idx0, x);
+    load_values<T, VPT, NT>(in + idx0, x);
   } else {
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
@@ -383,7 +411,7 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
     if (idx0 + VPT <= limit) {
       std::uint32_t w[VPT * U / 32 > 0 ? VPT * U / 32 : 1];
       pack_words<U, VPT>(code, w);
-      store_dwords(pay + idx0 * (U / 8), w);
+      store_dwords<NT>(pay + idx0 * (U / 8), w);
     } else {
       using wt = typename word<U>::type;
 #pragma unroll
@@ -420,7 +448,7 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
     std::uint8_t* dst = pay + k0 * p.CB;
     u32x4* d4 = reinterpret_cast<u32x4*>(dst);
     for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
-      __builtin_nontemporal_store(l4[c], d4 + c);
+      st<NT>(l4[c], d4 + c);
     }
     const std::uint32_t tail = nbytes & 15u;
     if (static_cast<std::uint32_t>(t) < tail) {
@@ -431,7 +459,24 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
   }
 }
 
-template<typename T, int U, int VPT, bool PACKED>
+// NB-byte code load / value store helpers of the chunked decompress
+template<bool NT, int NB>
+__device__ __forceinline__ void
+load_bytes(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[NB >= 4 ? NB / 4 : 1])
+{
+  if constexpr (NB == 2) {
+    w[0] = ld<NT>(reinterpret_cast<const std::uint16_t*>(src));
+  } else {
+    load_dwords<NT>(src, w);
+  }
+}
+
+// Decompress one tile.  Output mapping is chunk-interleaved: a chunk is 16
+// bytes of output (CPV = 16/sizeof(T) values) and chunk c of thread t holds
+// values [(c*WG + t)*CPV, +CPV) of the tile, so every store instruction of a
+// workgroup writes one contiguous 4 KB span and every aligned code load reads
+// one contiguous span as well.
+template<typename T, int U, int VPT, bool PACKED, bool NT>
 __device__ __forceinline__ void
 decompress_tile(const std::uint8_t* __restrict__ in,
                 typename fmt<T>::uw* __restrict__ out,
@@ -441,9 +486,12 @@ decompress_tile(const std::uint8_t* __restrict__ in,
 {
   using uw = typename fmt<T>::uw;
   constexpr int TILE = WG * VPT;
+  constexpr int CPV = 16 / static_cast<int>(sizeof(T));
+  constexpr int NCH = VPT / CPV;
+  constexpr int NB = CPV * U / 8; // code bytes per chunk
+  static_assert(NCH >= 1, "a thread needs at least one 16-byte chunk");
   const int t = static_cast<int>(threadIdx.x);
   const std::uint64_t base = tile * TILE;
-  const std::uint64_t idx0 = base + static_cast<std::uint64_t>(t) * VPT;
   const int B = 1 << p.lb;
   const std::uint8_t* pay = in + p.R;
 
@@ -457,7 +505,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
     u32x4* l4 = reinterpret_cast<u32x4*>(lds);
     for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
-      l4[c] = __builtin_nontemporal_load(s4 + c);
+      l4[c] = ld<NT>(s4 + c);
     }
     const std::uint32_t tail = nbytes & 15u;
     if (static_cast<std::uint32_t>(t) < tail) {
@@ -467,57 +515,73 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     __syncthreads();
   }
 
-  int E[VPT];
-  if (B >= VPT) {
-    const int e = idx0 < p.n ? load_block_exponent<U>(in, idx0 >> p.lb) : 0;
+  // phase 1: every exponent and code of the thread's chunks (all loads in
+  // flight together); phase 2: decode and store
+  const std::uint64_t limit = p.nb << p.lb; // payload words
+  int E[NCH][CPV];
+  uw code[NCH][CPV];
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      E[v] = e;
-    }
-  } else {
+  for (int c = 0; c < NCH; ++c) {
+    const std::uint32_t local0 = static_cast<std::uint32_t>((c * WG + t) * CPV);
+    const std::uint64_t i0 = base + local0;
+    if (B >= CPV) {
+      const int e = i0 < p.n ? load_block_exponent<U>(in, i0 >> p.lb) : 0;
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      E[v] = (idx0 + v < p.n) ? load_block_exponent<U>(in, (idx0 + v) >> p.lb) : 0;
-    }
-  }
-
-  uw code[VPT];
-  if (!PACKED) {
-    const std::uint64_t limit = p.nb << p.lb;
-    if (idx0 + VPT <= limit) {
-      std::uint32_t w[VPT * U / 32 > 0 ? VPT * U / 32 : 1];
-      load_dwords(pay + idx0 * (U / 8), w);
-      unpack_words<U, VPT>(w, code);
+      for (int v = 0; v < CPV; ++v) {
+        E[c][v] = e;
+      }
     } else {
-      using wt = typename word<U>::type;
 #pragma unroll
-      for (int v = 0; v < VPT; ++v) {
-        code[v] = (idx0 + v < limit) ? static_cast<uw>(reinterpret_cast<const wt*>(pay)[idx0 + v]) : uw{ 0 };
+      for (int v = 0; v < CPV; ++v) {
+        E[c][v] = (i0 + v < p.n) ? load_block_exponent<U>(in, (i0 + v) >> p.lb) : 0;
       }
     }
-  } else {
+    if (!PACKED) {
+      if (i0 + CPV <= limit) {
+        std::uint32_t w[NB >= 4 ? NB / 4 : 1];
+        load_bytes<NT, NB>(pay + i0 * (U / 8), w);
+        if constexpr (NB == 2) {
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      const std::uint32_t local = static_cast<std::uint32_t>(t * VPT + v);
-      const std::uint32_t kl = local >> p.lb;
-      const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
-      const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
-      code[v] = static_cast<uw>(extract_code<U>(lds, pos, p.bits));
+          for (int v = 0; v < CPV; ++v) {
+            code[c][v] = static_cast<uw>((w[0] >> (8 * v)) & 0xffu);
+          }
+        } else {
+          unpack_words<U, CPV>(w, code[c]);
+        }
+      } else {
+        using wt = typename word<U>::type;
+#pragma unroll
+        for (int v = 0; v < CPV; ++v) {
+          code[c][v] = (i0 + v < limit) ? static_cast<uw>(reinterpret_cast<const wt*>(pay)[i0 + v]) : uw{ 0 };
+        }
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < CPV; ++v) {
+        const std::uint32_t local = local0 + v;
+        const std::uint32_t kl = local >> p.lb;
+        const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
+        const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
+        code[c][v] = static_cast<uw>(extract_code<U>(lds, pos, p.bits));
+      }
     }
   }
-
-  uw y[VPT];
 #pragma unroll
-  for (int v = 0; v < VPT; ++v) {
-    y[v] = decode_value<T>(code[v], E[v], p.bits);
-  }
-  if (idx0 + VPT <= p.n) {
-    store_values<T, VPT>(out + idx0, y);
-  } else {
+  for (int c = 0; c < NCH; ++c) {
+    const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WG + t) * CPV);
+    uw y[CPV];
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      if (idx0 + v < p.n) {
-        out[idx0 + v] = y[v];
+    for (int v = 0; v < CPV; ++v) {
+      y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
+    }
+    if (i0 + CPV <= p.n) {
+      store_values<T, CPV, NT>(out + i0, y);
+    } else {
+#pragma unroll
+      for (int v = 0; v < CPV; ++v) {
+        if (i0 + v < p.n) {
+          out[i0 + v] = y[v];
+        }
       }
     }
   }

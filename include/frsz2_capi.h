@@ -116,6 +116,16 @@ int frsz2_gather(int dtype,
                  void* d_out,
                  frsz2_stream_t stream);
 
+/* Kernel variant of the tiled plain kernels for (op, dtype, bits): op 0 =
+ * compress, 1 = decompress; values_per_thread 4, 8 or 16 (a tile is 256 x
+ * that many values, so it also bounds the largest B the tiled path takes);
+ * nontemporal 0/1 puts the non-temporal hint on the streaming accesses.
+ * Process-wide; the batch autotuner sets it.  No counterpart in the
+ * reference (its kernels have a fixed launch shape,
+ * test/test_frsz_gpu.cu:325-330). */
+int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int nontemporal);
+int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* nontemporal);
+
 /* Host-memory variants: stage through device memory, run the HIP kernels and
  * copy the result back; synchronous.  They keep the host-in / host-out
  * meaning of the reference's CPU entry points and of the LibPressio plugin
