@@ -39,7 +39,92 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 25, help="values in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-arrays", type=int, default=4096)
+    ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--no-copy", action="store_true", help="skip the 1 GiB device-copy calibration")
     return ap.parse_args()
+
+
+def bench_batched(count, steps, seed=4):
+    """4096 arrays, lengths log-uniform over [1e3, 1.6e7], 60% float32 /
+    40% float64, bits from {16, 32}, B = 32: one autotuned persistent
+    graph launch vs a C++ loop of plain per-array launches, per direction."""
+    import numpy as np
+    import torch
+
+    import frsz_amd
+    from frsz_amd import batch
+    from frsz_amd.workloads import batched_lengths, smooth_signal_torch
+
+    lengths, is_f64, bits = batched_lengths(count, seed=seed)
+    B = 32
+    xs, streams, outs = [], [], []
+    alg = 0
+    for i, (n, f64, b) in enumerate(zip(lengths.tolist(), is_f64.tolist(), bits.tolist())):
+        dt = torch.float64 if f64 else torch.float32
+        x = smooth_signal_torch(n, seed=10_000 + i, dtype=dt)
+        size = frsz_amd.compressed_size(dt, b, B, n)
+        xs.append(x)
+        streams.append(torch.empty(size, dtype=torch.uint8, device="cuda"))
+        outs.append(torch.empty_like(x))
+        alg += n * x.element_size() + size
+    torch.cuda.synchronize()
+    cjobs = [batch.compress_job(x, s, b, B) for x, s, b in zip(xs, streams, bits.tolist())]
+    djobs = [batch.decompress_job(s, o, b, B) for s, o, b in zip(streams, outs, bits.tolist())]
+
+    def timed(plan, k):
+        plan.launch()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            plan.launch()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k * 1e-3
+
+    res = {"arrays": count, "total_values": int(lengths.sum()), "float64_share": round(float(is_f64.mean()), 3),
+           "algorithmic_bytes_per_direction": int(alg), "block": B}
+    # plain per-array launches from a C++ loop (the baseline), and the same captured in a graph
+    plain_c = batch.BatchPlan(cjobs, {"plain_launches": 1, "use_graph": 0})
+    plain_d = batch.BatchPlan(djobs, {"plain_launches": 1, "use_graph": 0})
+    t_pc = timed(plain_c, steps)
+    ref_streams = [s.clone() for s in streams]
+    t_pd = timed(plain_d, steps)
+    ref_outs_sum = None
+    plain_cg = batch.BatchPlan(cjobs, {"plain_launches": 1, "use_graph": 1})
+    t_pcg = timed(plain_cg, steps)
+    plan_c = batch.BatchPlan(cjobs)
+    tuned_c = plan_c.autotune(reps=2)
+    t_bc = timed(plan_c, steps)
+    same_c = all(torch.equal(a, b) for a, b in zip(streams, ref_streams))
+    del ref_streams
+    # decompress: plain outputs of each array compared with the batch's
+    plan_d = batch.BatchPlan(djobs)
+    tuned_d = plan_d.autotune(reps=2)
+    t_bd = timed(plan_d, steps)
+    same_d = True
+    for s, o, x, b in zip(streams, outs, xs, bits.tolist()):
+        r = frsz_amd.decompress(s, x.numel(), x.dtype, b, B)
+        same_d &= torch.equal(r.view(torch.uint8), o.view(torch.uint8))
+    torch.cuda.synchronize()
+    del ref_outs_sum
+    res.update({
+        "batched_compress_ms": round(t_bc * 1e3, 3), "plain_compress_ms": round(t_pc * 1e3, 3),
+        "plain_graph_compress_ms": round(t_pcg * 1e3, 3),
+        "batched_decompress_ms": round(t_bd * 1e3, 3), "plain_decompress_ms": round(t_pd * 1e3, 3),
+        "batched_compress_gbs": round(alg / t_bc / 1e9, 1), "batched_decompress_gbs": round(alg / t_bd / 1e9, 1),
+        "plain_compress_gbs": round(alg / t_pc / 1e9, 1), "plain_decompress_gbs": round(alg / t_pd / 1e9, 1),
+        "speedup_compress": round(t_pc / t_bc, 2), "speedup_decompress": round(t_pd / t_bd, 2),
+        "bit_exact_vs_plain": bool(same_c and same_d),
+        "autotuned": {"compress": tuned_c, "decompress": tuned_d},
+    })
+    for p_ in (plain_c, plain_d, plain_cg, plan_c, plan_d):
+        p_.close()
+    del xs, streams, outs
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(x_host, bits, block, sample):
@@ -191,6 +276,28 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
     }
+    if not args.no_copy:
+        # a plain 1 GiB device copy (torch), the calibration kernel for the
+        # HBM byte counters and a practical bandwidth reference
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(5):
+            b.copy_(a)
+        c1.record()
+        torch.cuda.synchronize()
+        tcopy = c0.elapsed_time(c1) / 5 * 1e-3
+        line["copy_calibration"] = {"bytes_read": a.numel() * 8, "bytes_written": a.numel() * 8,
+                                    "us": round(tcopy * 1e6, 1), "gbs": round(2 * a.numel() * 8 / tcopy / 1e9, 1)}
+        del a, b
+    del x, y, stream
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        print("bench: batched workload", file=sys.stderr, flush=True)
+        line["batched"] = bench_batched(args.batched_arrays, args.batched_steps)
     if not args.no_cpu:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(x_host, bits, block, min(args.cpu_sample, n))

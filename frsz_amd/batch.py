@@ -1,0 +1,131 @@
+"""Batched FRSZ2 over torch device tensors (C ABI frsz2_batch_*).
+
+    jobs = [batch.compress_job(x, stream_buf, bits=16, block=32), ...]
+    plan = batch.BatchPlan(jobs)          # descriptors uploaded, launch captured in a hipGraph
+    plan.autotune()                        # optional: time candidate launch shapes on these jobs
+    plan.launch()                          # one graph launch on the current stream
+
+Each job means exactly one frsz_amd.compress_into / decompress_into call
+(same validation, same output bytes).  The plan keeps references to the
+job tensors so they outlive it.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _capi
+from ._capi import check
+from .codec import compressed_size, dtype_code
+
+OP_COMPRESS = 0
+OP_DECOMPRESS = 1
+
+
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("op", ctypes.c_int),
+        ("dtype", ctypes.c_int),
+        ("bits", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("block", ctypes.c_uint64),
+        ("n", ctypes.c_uint64),
+        ("in_", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("in_bytes", ctypes.c_uint64),
+        ("out_bytes", ctypes.c_uint64),
+    ]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("workgroups", ctypes.c_int),
+        ("values_per_thread", ctypes.c_int),
+        ("tiles_per_claim", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+        ("plain_launches", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_L = _capi.lib
+_L.frsz2_batch_create.restype = ctypes.c_int
+_L.frsz2_batch_create.argtypes = [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.POINTER(BatchConfig),
+                                  ctypes.POINTER(ctypes.c_void_p)]
+_L.frsz2_batch_launch.restype = ctypes.c_int
+_L.frsz2_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+_L.frsz2_batch_autotune.restype = ctypes.c_int
+_L.frsz2_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(BatchConfig)]
+_L.frsz2_batch_get_config.restype = ctypes.c_int
+_L.frsz2_batch_get_config.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchConfig)]
+_L.frsz2_batch_set_config.restype = ctypes.c_int
+_L.frsz2_batch_set_config.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchConfig)]
+_L.frsz2_batch_destroy.restype = ctypes.c_int
+_L.frsz2_batch_destroy.argtypes = [ctypes.c_void_p]
+
+
+def compress_job(x: torch.Tensor, out: torch.Tensor, bits: int, block: int):
+    x = x.view(-1)
+    return (Job(OP_COMPRESS, dtype_code(x.dtype), bits, 0, block, x.numel(), x.data_ptr(), out.data_ptr(), 0,
+                out.numel()), (x, out))
+
+
+def decompress_job(stream: torch.Tensor, out: torch.Tensor, bits: int, block: int):
+    out_flat = out.view(-1)
+    return (Job(OP_DECOMPRESS, dtype_code(out.dtype), bits, 0, block, out_flat.numel(), stream.data_ptr(),
+                out_flat.data_ptr(), stream.numel(), 0), (stream, out))
+
+
+class BatchPlan:
+    def __init__(self, jobs, config: dict | None = None):
+        self._keep = [k for _, k in jobs]
+        arr = (Job * len(jobs))(*[j for j, _ in jobs])
+        handle = ctypes.c_void_p()
+        cfg = None
+        if config is not None:
+            cfg = BatchConfig(config.get("workgroups", 0), config.get("values_per_thread", 8),
+                              config.get("tiles_per_claim", 4), config.get("use_graph", 1),
+                              config.get("plain_launches", 0))
+        check(_L.frsz2_batch_create(arr, len(jobs), ctypes.byref(cfg) if cfg is not None else None,
+                                    ctypes.byref(handle)))
+        self._h = handle
+
+    def launch(self, stream: int | None = None) -> None:
+        s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        check(_L.frsz2_batch_launch(self._h, s))
+
+    def autotune(self, reps: int = 3) -> dict:
+        chosen = BatchConfig()
+        check(_L.frsz2_batch_autotune(self._h, torch.cuda.current_stream().cuda_stream, reps, ctypes.byref(chosen)))
+        return chosen.as_dict()
+
+    @property
+    def config(self) -> dict:
+        c = BatchConfig()
+        check(_L.frsz2_batch_get_config(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    @config.setter
+    def config(self, cfg: dict) -> None:
+        cur = self.config
+        cur.update(cfg)
+        c = BatchConfig(cur["workgroups"], cur["values_per_thread"], cur["tiles_per_claim"], cur["use_graph"],
+                        cur["plain_launches"])
+        check(_L.frsz2_batch_set_config(self._h, ctypes.byref(c)))
+
+    def close(self) -> None:
+        if self._h:
+            _L.frsz2_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["BatchPlan", "compress_job", "decompress_job", "compressed_size", "OP_COMPRESS", "OP_DECOMPRESS"]

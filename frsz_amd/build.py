@@ -85,7 +85,8 @@ def build_test_helpers() -> None:
           f"-I{INCLUDE}", "-o", os.path.join(nat, "libdevmath_host.so"), os.path.join(nat, "devmath_host.cpp")])
     api_src = os.path.join(nat, "test_header_api.cpp")
     if os.path.exists(api_src):
-        _run([hipcc(), "-std=c++17", "-O2", f"-I{INCLUDE}", "-o", os.path.join(nat, "test_header_api"), api_src,
+        _run([hipcc(), f"--offload-arch={ARCH}", "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang",
+              "-packed-fp32-ops", f"-I{INCLUDE}", "-o", os.path.join(nat, "test_header_api"), api_src,
               f"-L{LIBDIR}", "-lfrsz2_hip", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
 
 

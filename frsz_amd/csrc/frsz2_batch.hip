@@ -1,0 +1,628 @@
+// Batched FRSZ2: many independent arrays of uneven length, mixed float32 /
+// float64, bit widths and directions, in one persistent launch captured in a
+// hipGraph.
+//
+// The jobs' tiles form one global index space (job j owns tiles
+// [tile0_j, tile0_j + ntiles_j)).  A persistent grid of workgroups claims
+// tiles from a device counter (`chunk` tiles per atomic claim; chunk 0 =
+// static round-robin, no counter), finds the tile's job by binary search over
+// tile0 (starting from the previous job), and runs the same tile code as the
+// plain kernels for the job's class (direction, dtype, storage width U,
+// packed or not).  Jobs the tile path cannot take (B not a power of two or
+// larger than a tile, unaligned pointers) run as plain one-thread-per-block
+// launches inside the same graph.  The counter is reset by a one-thread
+// kernel captured at the head of the graph.
+//
+// The autotuner times candidate (workgroups, values per thread, chunk)
+// settings on the plan's own jobs and keeps the fastest.
+//
+// No counterpart in the reference: its kernels take one array per launch
+// (include/frsz2.hpp:1258-1288).
+#include "frsz2_capi.h"
+#include "frsz2_device.hpp"
+#include "frsz2_internal.hpp"
+#include "tile.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace frsz2 {
+
+using tile::tplan;
+using tile::WG;
+
+namespace {
+
+// device descriptor of one job (64 bytes)
+struct bjob
+{
+  const void* in;
+  void* out;
+  std::uint64_t n;
+  std::uint64_t nb;
+  std::uint64_t R;
+  std::uint64_t tile0;
+  std::uint32_t CB;
+  int bits;
+  int lb;
+  int cls;
+};
+static_assert(sizeof(bjob) == 64, "descriptor layout");
+
+// class = op * 14 + dtype_slot, dtype_slot in 0..13:
+//   float32: U8, U16, U32 x {aligned, packed} -> 0..5
+//   float64: U8, U16, U32, U64 x {aligned, packed} -> 6..13
+__host__ __device__ constexpr int
+class_of(int op, int dtype, int bits)
+{
+  const int U = storage_bits(bits);
+  const int ui = U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3;
+  const int packed = bits < U ? 1 : 0;
+  const int slot = dtype == FRSZ2_FLOAT32 ? ui * 2 + packed : 6 + ui * 2 + packed;
+  return op * 14 + slot;
+}
+
+// V32 / V64: values per thread of float32 / float64 tiles
+template<int V32, int V64>
+__device__ __forceinline__ void
+run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred)
+{
+  const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb, 0 };
+  auto* o8 = static_cast<std::uint8_t*>(J.out);
+  const auto* i8 = static_cast<const std::uint8_t*>(J.in);
+#define FRSZ2_V(T) (sizeof(T) == 4 ? V32 : V64)
+#define FRSZ2_C(T, U, PK)                                                                                    \
+  tile::compress_tile<T, U, FRSZ2_V(T), PK, true>(                                                           \
+    static_cast<const typename fmt<T>::uw*>(J.in), o8, p, t, lds, wred);                                     \
+  if (t == 0)                                                                                                \
+    tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);                                     \
+  break
+#define FRSZ2_D(T, U, PK)                                                                                    \
+  tile::decompress_tile<T, U, FRSZ2_V(T), PK, true>(i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds); \
+  break
+  switch (J.cls) {
+    case 0: FRSZ2_C(float, 8, false);
+    case 1: FRSZ2_C(float, 8, true);
+    case 2: FRSZ2_C(float, 16, false);
+    case 3: FRSZ2_C(float, 16, true);
+    case 4: FRSZ2_C(float, 32, false);
+    case 5: FRSZ2_C(float, 32, true);
+    case 6: FRSZ2_C(double, 8, false);
+    case 7: FRSZ2_C(double, 8, true);
+    case 8: FRSZ2_C(double, 16, false);
+    case 9: FRSZ2_C(double, 16, true);
+    case 10: FRSZ2_C(double, 32, false);
+    case 11: FRSZ2_C(double, 32, true);
+    case 12: FRSZ2_C(double, 64, false);
+    case 13: FRSZ2_C(double, 64, true);
+    case 14: FRSZ2_D(float, 8, false);
+    case 15: FRSZ2_D(float, 8, true);
+    case 16: FRSZ2_D(float, 16, false);
+    case 17: FRSZ2_D(float, 16, true);
+    case 18: FRSZ2_D(float, 32, false);
+    case 19: FRSZ2_D(float, 32, true);
+    case 20: FRSZ2_D(double, 8, false);
+    case 21: FRSZ2_D(double, 8, true);
+    case 22: FRSZ2_D(double, 16, false);
+    case 23: FRSZ2_D(double, 16, true);
+    case 24: FRSZ2_D(double, 32, false);
+    case 25: FRSZ2_D(double, 32, true);
+    case 26: FRSZ2_D(double, 64, false);
+    default: FRSZ2_D(double, 64, true);
+  }
+#undef FRSZ2_C
+#undef FRSZ2_D
+#undef FRSZ2_V
+}
+
+// job owning global tile g, searching from hint (jobs sorted by tile0)
+__device__ __forceinline__ int
+find_job(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, int hint)
+{
+  if (g >= jobs[hint].tile0 && (hint + 1 == njobs || g < jobs[hint + 1].tile0)) {
+    return hint;
+  }
+  int lo = 0;
+  int hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 <= g) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+template<int V32, int V64>
+__global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
+                                              int njobs,
+                                              std::uint64_t total_tiles,
+                                              unsigned long long* counter,
+                                              int chunk)
+{
+  __shared__ std::uint32_t lds[tile::lds_dwords<(V32 > V64 ? V32 : V64)>()];
+  __shared__ std::uint32_t wred[WG / tile::WAVE];
+  __shared__ unsigned long long claim;
+  int j = 0;
+  std::uint64_t next = blockIdx.x;
+  for (;;) {
+    std::uint64_t g0;
+    std::uint64_t g1;
+    if (chunk > 0) {
+      if (threadIdx.x == 0) {
+        claim = atomicAdd(counter, static_cast<unsigned long long>(chunk));
+      }
+      __syncthreads();
+      g0 = claim;
+      __syncthreads();
+      g1 = g0 + static_cast<std::uint64_t>(chunk);
+    } else {
+      g0 = next;
+      g1 = g0 + 1;
+      next += gridDim.x;
+    }
+    if (g0 >= total_tiles) {
+      break;
+    }
+    if (g1 > total_tiles) {
+      g1 = total_tiles;
+    }
+    for (std::uint64_t g = g0; g < g1; ++g) {
+      j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, j));
+      const bjob J = jobs[j];
+      run_tile<V32, V64>(J, g - J.tile0, lds, wred);
+    }
+  }
+}
+
+__global__ void
+k_reset_counter(unsigned long long* counter)
+{
+  *counter = 0ull;
+}
+
+int
+ilog2_exact(std::uint64_t B)
+{
+  if (B == 0 || (B & (B - 1)) != 0) {
+    return -1;
+  }
+  int l = 0;
+  while ((std::uint64_t{ 1 } << l) < B) {
+    ++l;
+  }
+  return l;
+}
+
+bool
+aligned16(const void* p)
+{
+  return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0;
+}
+
+} // namespace
+
+} // namespace frsz2
+
+struct frsz2_batch_plan
+{
+  std::vector<frsz2_job> jobs;         // as given
+  std::vector<int> tiled;              // job index -> 1 if it takes the tile path
+  std::vector<frsz2::bjob> host_desc;  // per values-per-thread setting, rebuilt on change
+  frsz2::bjob* d_desc = nullptr;
+  unsigned long long* d_counter = nullptr;
+  int ndesc = 0;
+  std::uint64_t total_tiles = 0;
+  frsz2_batch_config cfg{};
+  int cus = 256;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace frsz2 {
+namespace {
+
+int
+validate_job(const frsz2_job& j, int idx)
+{
+  if (j.op != FRSZ2_OP_COMPRESS && j.op != FRSZ2_OP_DECOMPRESS) {
+    return set_error(FRSZ2_EARG, "job " + std::to_string(idx) + ": op must be compress or decompress");
+  }
+  int rc = check_params(j.dtype, j.bits, j.block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(j.bits, j.block, j.n);
+  if (j.op == FRSZ2_OP_COMPRESS && j.out_bytes < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, "job " + std::to_string(idx) + ": output capacity below stream size");
+  }
+  if (j.op == FRSZ2_OP_DECOMPRESS && j.in_bytes < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, "job " + std::to_string(idx) + ": input stream below stream size");
+  }
+  const std::uintptr_t vs = j.dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  const void* vals = j.op == FRSZ2_OP_COMPRESS ? j.in : j.out;
+  const void* strm = j.op == FRSZ2_OP_COMPRESS ? j.out : j.in;
+  if (!strm || (j.n > 0 && !vals) || (reinterpret_cast<std::uintptr_t>(vals) & (vs - 1)) ||
+      (reinterpret_cast<std::uintptr_t>(strm) & 7u)) {
+    return set_error(FRSZ2_EARG, "job " + std::to_string(idx) + ": null or misaligned pointer");
+  }
+  return FRSZ2_OK;
+}
+
+// float64 tiles always use this many values per thread (measured best)
+constexpr int V64 = 4;
+
+// (re)build the device descriptors for cfg.values_per_thread (float32)
+int
+build_descriptors(frsz2_batch_plan* P)
+{
+  P->host_desc.clear();
+  P->tiled.assign(P->jobs.size(), 0);
+  std::uint64_t tiles = 0;
+  for (std::size_t i = 0; i < P->jobs.size(); ++i) {
+    const frsz2_job& j = P->jobs[i];
+    const std::uint64_t TILE =
+      static_cast<std::uint64_t>(WG) * (j.dtype == FRSZ2_FLOAT32 ? P->cfg.values_per_thread : V64);
+    const int lb = ilog2_exact(j.block);
+    if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
+      continue;
+    }
+    const layout L = layout::make(j.bits, j.block, j.n);
+    std::uint64_t nt = (j.n + TILE - 1) / TILE;
+    if (j.op == FRSZ2_OP_COMPRESS && nt == 0) {
+      nt = 1; // the pad of an empty stream is still written
+    }
+    if (nt == 0) {
+      continue;
+    }
+    P->tiled[i] = 1;
+    bjob d{};
+    d.in = j.in;
+    d.out = j.out;
+    d.n = j.n;
+    d.nb = L.nb;
+    d.R = L.R;
+    d.tile0 = tiles;
+    d.CB = static_cast<std::uint32_t>(L.CB);
+    d.bits = j.bits;
+    d.lb = lb;
+    d.cls = class_of(j.op, j.dtype, j.bits);
+    P->host_desc.push_back(d);
+    tiles += nt;
+  }
+  P->total_tiles = tiles;
+  if (P->d_desc) {
+    (void)hipFree(P->d_desc);
+    P->d_desc = nullptr;
+  }
+  P->ndesc = static_cast<int>(P->host_desc.size());
+  if (P->ndesc > 0) {
+    hipError_t e = hipMalloc(&P->d_desc, P->host_desc.size() * sizeof(bjob));
+    if (e != hipSuccess) {
+      return hip_status(e, "hipMalloc");
+    }
+    e = hipMemcpy(P->d_desc, P->host_desc.data(), P->host_desc.size() * sizeof(bjob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      return hip_status(e, "hipMemcpy");
+    }
+  }
+  return FRSZ2_OK;
+}
+
+// enqueue the whole batch on s (also used under capture)
+int
+enqueue(frsz2_batch_plan* P, hipStream_t s)
+{
+  if (P->cfg.plain_launches) {
+    for (const frsz2_job& j : P->jobs) {
+      const int rc = j.op == FRSZ2_OP_COMPRESS
+                       ? compress_device(j.dtype, j.bits, j.block, j.in, j.n, j.out, j.out_bytes, s)
+                       : decompress_device(j.dtype, j.bits, j.block, j.in, j.in_bytes, j.n, j.out, s);
+      if (rc) {
+        return rc;
+      }
+    }
+    return FRSZ2_OK;
+  }
+  if (P->ndesc > 0) {
+    const int chunk = P->cfg.tiles_per_claim;
+    std::uint64_t grid = static_cast<std::uint64_t>(P->cfg.workgroups);
+    if (chunk == 0 && grid > P->total_tiles) {
+      grid = P->total_tiles;
+    }
+    if (chunk > 0) {
+      hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, P->d_counter);
+    }
+    if (P->cfg.values_per_thread == 8) {
+      hipLaunchKernelGGL((k_batch<8, V64>), dim3(static_cast<unsigned>(grid)), dim3(WG), 0, s, P->d_desc, P->ndesc,
+                         P->total_tiles, P->d_counter, chunk);
+    } else {
+      hipLaunchKernelGGL((k_batch<4, V64>), dim3(static_cast<unsigned>(grid)), dim3(WG), 0, s, P->d_desc, P->ndesc,
+                         P->total_tiles, P->d_counter, chunk);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      return hip_status(e, "batch launch");
+    }
+  }
+  // jobs outside the tile path: plain launches on the same stream
+  for (std::size_t i = 0; i < P->jobs.size(); ++i) {
+    if (P->tiled[i]) {
+      continue;
+    }
+    const frsz2_job& j = P->jobs[i];
+    int rc;
+    if (j.op == FRSZ2_OP_COMPRESS) {
+      rc = compress_device(j.dtype, j.bits, j.block, j.in, j.n, j.out, j.out_bytes, s);
+    } else {
+      rc = decompress_device(j.dtype, j.bits, j.block, j.in, j.in_bytes, j.n, j.out, s);
+    }
+    if (rc) {
+      return rc;
+    }
+  }
+  return FRSZ2_OK;
+}
+
+void
+drop_graph(frsz2_batch_plan* P)
+{
+  if (P->exec) {
+    (void)hipGraphExecDestroy(P->exec);
+    P->exec = nullptr;
+  }
+  if (P->graph) {
+    (void)hipGraphDestroy(P->graph);
+    P->graph = nullptr;
+  }
+}
+
+int
+capture_graph(frsz2_batch_plan* P)
+{
+  drop_graph(P);
+  if (!P->cfg.use_graph) {
+    return FRSZ2_OK;
+  }
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    return hip_status(e, "hipStreamCreate");
+  }
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = e == hipSuccess ? enqueue(P, cs) : hip_status(e, "hipStreamBeginCapture");
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(cs, &g);
+  if (rc == FRSZ2_OK && e2 != hipSuccess) {
+    rc = hip_status(e2, "hipStreamEndCapture");
+  }
+  if (rc == FRSZ2_OK) {
+    P->graph = g;
+    e = hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      rc = hip_status(e, "hipGraphInstantiate");
+    }
+  } else if (g) {
+    (void)hipGraphDestroy(g);
+  }
+  (void)hipStreamDestroy(cs);
+  return rc;
+}
+
+int
+apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
+{
+  frsz2_batch_config n = c;
+  if (n.values_per_thread != 4 && n.values_per_thread != 8) {
+    n.values_per_thread = 8;
+  }
+  if (n.workgroups <= 0) {
+    n.workgroups = 4 * P->cus;
+  }
+  if (n.tiles_per_claim < 0) {
+    n.tiles_per_claim = 0;
+  }
+  const bool rebuild = n.values_per_thread != P->cfg.values_per_thread || P->host_desc.empty();
+  P->cfg = n;
+  if (rebuild) {
+    const int rc = build_descriptors(P);
+    if (rc) {
+      return rc;
+    }
+  }
+  return capture_graph(P);
+}
+
+void
+destroy(frsz2_batch_plan* P)
+{
+  drop_graph(P);
+  if (P->d_desc) {
+    (void)hipFree(P->d_desc);
+  }
+  if (P->d_counter) {
+    (void)hipFree(P->d_counter);
+  }
+  delete P;
+}
+
+} // namespace
+} // namespace frsz2
+
+extern "C" {
+
+int
+frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config* cfg, frsz2_batch_plan** plan)
+{
+  using namespace frsz2;
+  clear_error();
+  if (!plan || (njobs > 0 && !jobs)) {
+    return set_error(FRSZ2_EARG, "null pointer");
+  }
+  *plan = nullptr;
+  for (size_t i = 0; i < njobs; ++i) {
+    const int rc = validate_job(jobs[i], static_cast<int>(i));
+    if (rc) {
+      return rc;
+    }
+  }
+  auto* P = new frsz2_batch_plan();
+  P->jobs.assign(jobs, jobs + njobs);
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) {
+      P->cus = cus;
+    }
+  }
+  hipError_t e = hipMalloc(&P->d_counter, sizeof(unsigned long long));
+  if (e != hipSuccess) {
+    destroy(P);
+    return hip_status(e, "hipMalloc");
+  }
+  frsz2_batch_config c{};
+  if (cfg) {
+    c = *cfg;
+  } else {
+    c.workgroups = 0;
+    c.values_per_thread = 8;
+    c.tiles_per_claim = 4;
+    c.use_graph = 1;
+    c.plain_launches = 0;
+  }
+  P->cfg.values_per_thread = 0; // force a descriptor build
+  const int rc = apply_config(P, c);
+  if (rc) {
+    destroy(P);
+    return rc;
+  }
+  *plan = P;
+  return FRSZ2_OK;
+}
+
+int
+frsz2_batch_launch(frsz2_batch_plan* plan, frsz2_stream_t stream)
+{
+  using namespace frsz2;
+  clear_error();
+  if (!plan) {
+    return set_error(FRSZ2_EARG, "null plan");
+  }
+  const auto s = static_cast<hipStream_t>(stream);
+  if (plan->exec) {
+    return hip_status(hipGraphLaunch(plan->exec, s), "hipGraphLaunch");
+  }
+  return enqueue(plan, s);
+}
+
+int
+frsz2_batch_get_config(const frsz2_batch_plan* plan, frsz2_batch_config* cfg)
+{
+  frsz2::clear_error();
+  if (!plan || !cfg) {
+    return frsz2::set_error(FRSZ2_EARG, "null pointer");
+  }
+  *cfg = plan->cfg;
+  return FRSZ2_OK;
+}
+
+int
+frsz2_batch_set_config(frsz2_batch_plan* plan, const frsz2_batch_config* cfg)
+{
+  frsz2::clear_error();
+  if (!plan || !cfg) {
+    return frsz2::set_error(FRSZ2_EARG, "null pointer");
+  }
+  return frsz2::apply_config(plan, *cfg);
+}
+
+int
+frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, frsz2_batch_config* chosen)
+{
+  using namespace frsz2;
+  clear_error();
+  if (!plan) {
+    return set_error(FRSZ2_EARG, "null plan");
+  }
+  if (reps <= 0) {
+    reps = 3;
+  }
+  const auto s = static_cast<hipStream_t>(stream);
+  const int use_graph = plan->cfg.use_graph;
+  hipEvent_t e0 = nullptr;
+  hipEvent_t e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    return set_error(FRSZ2_EHIP, "hipEventCreate failed");
+  }
+  frsz2_batch_config best = plan->cfg;
+  float best_ms = -1.0f;
+  const int vpts[2] = { 4, 8 };
+  const int wg_per_cu[4] = { 2, 4, 8, 16 };
+  const int chunks[4] = { 0, 1, 4, 16 };
+  int rc = FRSZ2_OK;
+  for (int vi = 0; vi < 2 && rc == FRSZ2_OK; ++vi) {
+    for (int wi = 0; wi < 4 && rc == FRSZ2_OK; ++wi) {
+      for (int ci = 0; ci < 4 && rc == FRSZ2_OK; ++ci) {
+        frsz2_batch_config c{};
+        c.values_per_thread = vpts[vi];
+        c.workgroups = wg_per_cu[wi] * plan->cus;
+        c.tiles_per_claim = chunks[ci];
+        c.use_graph = 0;
+        c.plain_launches = 0;
+        rc = apply_config(plan, c);
+        if (rc) {
+          break;
+        }
+        rc = enqueue(plan, s); // warm-up
+        if (rc) {
+          break;
+        }
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && rc == FRSZ2_OK; ++r) {
+          rc = enqueue(plan, s);
+        }
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = set_error(FRSZ2_EHIP, "hipEventSynchronize failed");
+          break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (best_ms < 0.0f || ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  best.use_graph = use_graph;
+  const int rc2 = apply_config(plan, best);
+  if (rc == FRSZ2_OK) {
+    rc = rc2;
+  }
+  if (chosen) {
+    *chosen = plan->cfg;
+  }
+  return rc;
+}
+
+int
+frsz2_batch_destroy(frsz2_batch_plan* plan)
+{
+  frsz2::clear_error();
+  if (plan) {
+    frsz2::destroy(plan);
+  }
+  return FRSZ2_OK;
+}
+
+} // extern "C"

@@ -471,11 +471,15 @@ load_bytes(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[NB >= 4 ? NB
   }
 }
 
-// Decompress one tile.  Output mapping is chunk-interleaved: a chunk is 16
-// bytes of output (CPV = 16/sizeof(T) values) and chunk c of thread t holds
-// values [(c*WG + t)*CPV, +CPV) of the tile, so every store instruction of a
-// workgroup writes one contiguous 4 KB span and every aligned code load reads
-// one contiguous span as well.
+// Decompress one tile.  A thread's VPT values are split into NCH chunks of
+// VC consecutive values, chunk c of thread t holding values
+// [(c*WG + t)*VC, +VC) of the tile.  With codes read from global memory
+// (bits == U) a chunk is 16 bytes of codes (VC = 128/U, at most VPT), so
+// every code load is one 128-bit access and, when a thread has several
+// chunks, each load / store instruction of the workgroup covers one
+// contiguous span.  With codes staged in LDS (bits < U) a thread has one
+// chunk of VPT consecutive values.  All loads of a thread are issued before
+// any value is decoded.
 template<typename T, int U, int VPT, bool PACKED, bool NT>
 __device__ __forceinline__ void
 decompress_tile(const std::uint8_t* __restrict__ in,
@@ -486,10 +490,10 @@ decompress_tile(const std::uint8_t* __restrict__ in,
 {
   using uw = typename fmt<T>::uw;
   constexpr int TILE = WG * VPT;
-  constexpr int CPV = 16 / static_cast<int>(sizeof(T));
-  constexpr int NCH = VPT / CPV;
-  constexpr int NB = CPV * U / 8; // code bytes per chunk
-  static_assert(NCH >= 1, "a thread needs at least one 16-byte chunk");
+  constexpr int VC = PACKED ? VPT : (VPT < 128 / U ? VPT : 128 / U);
+  constexpr int NCH = VPT / VC;
+  constexpr int NB = VC * U / 8; // code bytes per chunk
+  static_assert(VC * static_cast<int>(sizeof(T)) % 16 == 0, "a chunk is whole 16-byte stores");
   const int t = static_cast<int>(threadIdx.x);
   const std::uint64_t base = tile * TILE;
   const int B = 1 << p.lb;
@@ -515,49 +519,47 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     __syncthreads();
   }
 
-  // phase 1: every exponent and code of the thread's chunks (all loads in
-  // flight together); phase 2: decode and store
   const std::uint64_t limit = p.nb << p.lb; // payload words
-  int E[NCH][CPV];
-  uw code[NCH][CPV];
+  int E[NCH][VC];
+  uw code[NCH][VC];
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
-    const std::uint32_t local0 = static_cast<std::uint32_t>((c * WG + t) * CPV);
+    const std::uint32_t local0 = static_cast<std::uint32_t>((c * WG + t) * VC);
     const std::uint64_t i0 = base + local0;
-    if (B >= CPV) {
+    if (B >= VC) {
       const int e = i0 < p.n ? load_block_exponent<U>(in, i0 >> p.lb) : 0;
 #pragma unroll
-      for (int v = 0; v < CPV; ++v) {
+      for (int v = 0; v < VC; ++v) {
         E[c][v] = e;
       }
     } else {
 #pragma unroll
-      for (int v = 0; v < CPV; ++v) {
+      for (int v = 0; v < VC; ++v) {
         E[c][v] = (i0 + v < p.n) ? load_block_exponent<U>(in, (i0 + v) >> p.lb) : 0;
       }
     }
     if (!PACKED) {
-      if (i0 + CPV <= limit) {
+      if (i0 + VC <= limit) {
         std::uint32_t w[NB >= 4 ? NB / 4 : 1];
         load_bytes<NT, NB>(pay + i0 * (U / 8), w);
         if constexpr (NB == 2) {
 #pragma unroll
-          for (int v = 0; v < CPV; ++v) {
+          for (int v = 0; v < VC; ++v) {
             code[c][v] = static_cast<uw>((w[0] >> (8 * v)) & 0xffu);
           }
         } else {
-          unpack_words<U, CPV>(w, code[c]);
+          unpack_words<U, VC>(w, code[c]);
         }
       } else {
         using wt = typename word<U>::type;
 #pragma unroll
-        for (int v = 0; v < CPV; ++v) {
+        for (int v = 0; v < VC; ++v) {
           code[c][v] = (i0 + v < limit) ? static_cast<uw>(reinterpret_cast<const wt*>(pay)[i0 + v]) : uw{ 0 };
         }
       }
     } else {
 #pragma unroll
-      for (int v = 0; v < CPV; ++v) {
+      for (int v = 0; v < VC; ++v) {
         const std::uint32_t local = local0 + v;
         const std::uint32_t kl = local >> p.lb;
         const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
@@ -568,17 +570,17 @@ decompress_tile(const std::uint8_t* __restrict__ in,
   }
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
-    const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WG + t) * CPV);
-    uw y[CPV];
+    const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WG + t) * VC);
+    uw y[VC];
 #pragma unroll
-    for (int v = 0; v < CPV; ++v) {
+    for (int v = 0; v < VC; ++v) {
       y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
     }
-    if (i0 + CPV <= p.n) {
-      store_values<T, CPV, NT>(out + i0, y);
+    if (i0 + VC <= p.n) {
+      store_values<T, VC, NT>(out + i0, y);
     } else {
 #pragma unroll
-      for (int v = 0; v < CPV; ++v) {
+      for (int v = 0; v < VC; ++v) {
         if (i0 + v < p.n) {
           out[i0 + v] = y[v];
         }

@@ -126,6 +126,60 @@ int frsz2_gather(int dtype,
 int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int nontemporal);
 int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* nontemporal);
 
+/* ---- batched: many arrays in one persistent, graph-captured launch ----
+ * No counterpart in the reference: its kernels take one array per launch
+ * (include/frsz2.hpp:1258-1288).  Each job has the meaning of one
+ * frsz2_compress / frsz2_decompress call (same checks, same bytes). */
+
+enum
+{
+  FRSZ2_OP_COMPRESS = 0,
+  FRSZ2_OP_DECOMPRESS = 1
+};
+
+typedef struct frsz2_job
+{
+  int op;             /* FRSZ2_OP_COMPRESS or FRSZ2_OP_DECOMPRESS */
+  int dtype;          /* FRSZ2_FLOAT32 / FRSZ2_FLOAT64 */
+  int bits;
+  int reserved;
+  uint64_t block;     /* exponent block size B */
+  uint64_t n;         /* values */
+  const void* in;     /* compress: values; decompress: stream (device memory) */
+  void* out;          /* compress: stream; decompress: values (device memory) */
+  uint64_t in_bytes;  /* decompress: bytes available at in */
+  uint64_t out_bytes; /* compress: capacity at out */
+} frsz2_job;
+
+typedef struct frsz2_batch_config
+{
+  int workgroups;        /* persistent grid size (0 = 4 per CU) */
+  int values_per_thread; /* 4 or 8 for float32 tiles (float64 tiles use 4) */
+  int tiles_per_claim;   /* tiles per atomic claim; 0 = static round-robin */
+  int use_graph;         /* 1 = launch through a captured hipGraph */
+  int plain_launches;    /* 1 = no persistent kernel: one plain frsz2_compress /
+                            frsz2_decompress launch per job, in job order (the
+                            per-array baseline the batch is measured against) */
+} frsz2_batch_config;
+
+typedef struct frsz2_batch_plan frsz2_batch_plan;
+
+/* Validate the jobs, upload their descriptors and capture the launch.
+ * cfg NULL = defaults (4 workgroups per CU, 8 values per thread, 4 tiles per
+ * claim, graph on).  Jobs must not overlap in memory. */
+int frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config* cfg,
+                       frsz2_batch_plan** plan);
+/* Run every job of the plan, stream-ordered.  Launches of one plan must not
+ * overlap in time (they share the plan's tile counter). */
+int frsz2_batch_launch(frsz2_batch_plan* plan, frsz2_stream_t stream);
+/* Time candidate configurations on the plan's jobs (each launched reps
+ * times on stream; outputs are overwritten with identical bytes), keep the
+ * fastest and re-capture; the choice is written to *chosen if not NULL. */
+int frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, frsz2_batch_config* chosen);
+int frsz2_batch_get_config(const frsz2_batch_plan* plan, frsz2_batch_config* cfg);
+int frsz2_batch_set_config(frsz2_batch_plan* plan, const frsz2_batch_config* cfg);
+int frsz2_batch_destroy(frsz2_batch_plan* plan);
+
 /* Host-memory variants: stage through device memory, run the HIP kernels and
  * copy the result back; synchronous.  They keep the host-in / host-out
  * meaning of the reference's CPU entry points and of the LibPressio plugin

@@ -1,0 +1,159 @@
+// Exercises the drop-in C++ header (include/frsz2.hpp) the way the
+// reference's GPU test drives its own header (test/test_frsz_gpu.cu:316-410):
+// the 9-value repeating pattern, double n = 2049 and float n = 2160, with the
+// same (bits, B) list, compress + decompress, lossless round trip.  Also the
+// device-pointer methods, the in-kernel decompress_gpu_element accessor and
+// the runtime dispatch with its error messages.
+//
+// Usage: test_header_api <outdir>
+//   writes <outdir>/<float|double>_b<bits>_B<B>.bin (the stream bytes) for
+//   pytest to compare with the golden vectors; prints "FAIL ..." lines and
+//   exits non-zero on any failed check.
+#include "frsz2.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+int g_fail = 0;
+
+#define CHECK(cond, ...)                                                                                     \
+  do {                                                                                                       \
+    if (!(cond)) {                                                                                           \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__);                                                       \
+      std::printf(__VA_ARGS__);                                                                              \
+      std::printf("\n");                                                                                     \
+      ++g_fail;                                                                                              \
+    }                                                                                                        \
+  } while (0)
+
+#define HIP_OK(expr) CHECK((expr) == hipSuccess, "%s", #expr)
+
+template<class C>
+__global__ void
+element_kernel(C view, std::size_t n, typename C::fp_type* out)
+{
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[i] = view.decompress_gpu_element(i);
+  }
+}
+
+template<int bits, int B, class T>
+void
+run_case(const std::vector<T>& in, const std::string& outdir)
+{
+  using C = frsz::frsz2_compressor<bits, B, T>;
+  const std::size_t n = in.size();
+  const std::size_t size = C::compute_compressed_memory_size_byte(n);
+  const char* tname = sizeof(T) == 4 ? "float" : "double";
+
+  // host path
+  std::vector<std::uint8_t> stream(size, 0xFF);
+  C host_view(stream.data(), n);
+  CHECK(host_view.compress_cpu_impl(in.data()) == 0, "compress_cpu_impl");
+  std::vector<T> out(n, T(-1));
+  CHECK(host_view.decompress_cpu_impl(out.data()) == 0, "decompress_cpu_impl");
+  CHECK(std::memcmp(out.data(), in.data(), n * sizeof(T)) == 0, "%s b%d B%d: round trip not lossless", tname, bits,
+        B);
+  for (std::size_t i : { std::size_t{ 0 }, n / 3, n - 1 }) {
+    const T e = host_view.decompress_cpu_element(i);
+    CHECK(std::memcmp(&e, &out[i], sizeof(T)) == 0, "decompress_cpu_element(%zu)", i);
+  }
+  const std::string path = outdir + "/" + tname + "_b" + std::to_string(bits) + "_B" + std::to_string(B) + ".bin";
+  if (FILE* f = std::fopen(path.c_str(), "wb")) {
+    std::fwrite(stream.data(), 1, size, f);
+    std::fclose(f);
+  }
+
+  // device path: same bytes, same values
+  T* d_in = nullptr;
+  T* d_out = nullptr;
+  std::uint8_t* d_stream = nullptr;
+  HIP_OK(hipMalloc(&d_in, n * sizeof(T)));
+  HIP_OK(hipMalloc(&d_out, n * sizeof(T)));
+  HIP_OK(hipMalloc(&d_stream, size));
+  HIP_OK(hipMemcpy(d_in, in.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  C dev_view(d_stream, n);
+  dev_view.compress_gpu(d_in);
+  std::vector<std::uint8_t> dstream(size);
+  HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+  CHECK(dstream == stream, "%s b%d B%d: device stream differs from host-API stream", tname, bits, B);
+  dev_view.decompress_gpu(d_out);
+  std::vector<T> dout(n);
+  HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+  CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "decompress_gpu differs");
+  // in-kernel accessor
+  HIP_OK(hipMemset(d_out, 0, n * sizeof(T)));
+  element_kernel<C><<<dim3((n + 255) / 256), dim3(256)>>>(dev_view, n, d_out);
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+  CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "decompress_gpu_element differs");
+  HIP_OK(hipFree(d_in));
+  HIP_OK(hipFree(d_out));
+  HIP_OK(hipFree(d_stream));
+}
+
+void
+check_dispatch()
+{
+  std::vector<float> x(100, 1.5f);
+  using C = frsz::frsz2_compressor<16, 32, float>;
+  std::vector<std::uint8_t> s(C::compute_compressed_memory_size_byte(x.size()));
+  using bits = frsz::int_list_t<8, 16, 32>;
+  using exps = frsz::int_list_t<2, 4, 8, 16, 32, 64, 128, 256, 512, 1024>;
+  frsz::dispatch_frsz2_compression<float>(bits{}, 16, exps{}, 32, x.data(), x.size(), s.data());
+  std::vector<float> y(x.size());
+  frsz::dispatch_frsz2_decompression<float>(bits{}, 16, exps{}, 32, y.data(), y.size(), s.data());
+  CHECK(y == x, "dispatch round trip");
+  std::string msg;
+  try {
+    frsz::dispatch_frsz2_compression<float>(bits{}, 12, exps{}, 32, x.data(), x.size(), s.data());
+  } catch (std::runtime_error const& e) {
+    msg = e.what();
+  }
+  CHECK(msg == "Unsupported number of bits: 12", "bits message: %s", msg.c_str());
+  msg.clear();
+  try {
+    frsz::dispatch_frsz2_compression<float>(bits{}, 16, exps{}, 3, x.data(), x.size(), s.data());
+  } catch (std::runtime_error const& e) {
+    msg = e.what();
+  }
+  CHECK(msg == "Unsupported exponent block size: 3", "block message: %s", msg.c_str());
+}
+
+} // namespace
+
+int
+main(int argc, char** argv)
+{
+  const std::string outdir = argc > 1 ? argv[1] : ".";
+  const double pat[9] = { 1., 2., 3., 4., 0.25, -0.25, -0.125, 1 / 32., 0.125 };
+  std::vector<double> vd(2049);
+  for (std::size_t i = 0; i < vd.size(); ++i) {
+    vd[i] = pat[i % 9];
+  }
+  std::vector<float> vf(2049 + 111);
+  for (std::size_t i = 0; i < vf.size(); ++i) {
+    vf[i] = static_cast<float>(pat[i % 9]);
+  }
+  run_case<32, 32, double>(vd, outdir);
+  run_case<16, 32, double>(vd, outdir);
+  run_case<16, 8, double>(vd, outdir);
+  run_case<15, 8, double>(vd, outdir);
+  run_case<9, 8, double>(vd, outdir);
+  run_case<9, 4, double>(vd, outdir);
+  run_case<9, 5, double>(vd, outdir);
+  run_case<16, 8, float>(vf, outdir);
+  run_case<15, 8, float>(vf, outdir);
+  run_case<9, 4, float>(vf, outdir);
+  run_case<9, 5, float>(vf, outdir);
+  check_dispatch();
+  std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "OK", g_fail);
+  return g_fail ? 1 : 0;
+}

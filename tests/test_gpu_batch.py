@@ -1,0 +1,124 @@
+"""Batched path: every job of a persistent, graph-captured batch must produce
+exactly the bytes of the plain single-array call (which the plain tests pin
+to the oracle), for any launch configuration and on every graph replay."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def fz():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frsz_amd
+    from frsz_amd import batch
+
+    return frsz_amd, batch
+
+
+def make_arrays(seed, count=60):
+    from frsz_amd.workloads import smooth_signal_torch
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        f64 = rng.random() < 0.4
+        dt = torch.float64 if f64 else torch.float32
+        W = 64 if f64 else 32
+        bits = int(rng.choice([4, 8, 9, 12, 16, 21, 32] + ([40, 52, 64] if f64 else [])))
+        if f64 and bits <= 8:
+            bits = 16
+        B = int(rng.choice([1, 2, 4, 16, 32, 64, 256, 1024, 5, 100]))
+        n = int(np.exp(rng.uniform(0, np.log(300_000)))) if i % 10 else int(rng.choice([0, 1, 31]))
+        x = smooth_signal_torch(max(n, 1), seed=1000 + i, dtype=dt)[:n].contiguous()
+        x.mul_(float(2.0 ** rng.integers(-20, 20)))
+        specs.append((x, bits, B))
+        assert bits <= W
+    return specs
+
+
+def plain_outputs(fz_mod, specs):
+    streams, values = [], []
+    for x, bits, B in specs:
+        s = fz_mod.compress(x, bits, B)
+        streams.append(s)
+        values.append(fz_mod.decompress(s, x.numel(), x.dtype, bits, B))
+    return streams, values
+
+
+CONFIGS = [None,
+           {"workgroups": 64, "values_per_thread": 4, "tiles_per_claim": 0, "use_graph": 1},
+           {"workgroups": 1024, "values_per_thread": 8, "tiles_per_claim": 1, "use_graph": 0},
+           {"workgroups": 512, "values_per_thread": 8, "tiles_per_claim": 16, "use_graph": 1},
+           {"plain_launches": 1, "use_graph": 1}]
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "static", "nograph", "chunk16", "plain-graph"])
+def test_batch_matches_plain(fz, cfg):
+    fz_mod, batch = fz
+    specs = make_arrays(1)
+    ref_s, ref_v = plain_outputs(fz_mod, specs)
+    outs = [torch.full_like(s, 0x5A) for s in ref_s]
+    plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
+    plan.launch()
+    vals = [torch.full_like(x, float("nan")) for x, _, _ in specs]
+    dplan = batch.BatchPlan([batch.decompress_job(s, v, b, B) for (x, b, B), s, v in zip(specs, ref_s, vals)], cfg)
+    dplan.launch()
+    torch.cuda.synchronize()
+    for i, ((x, b, B), o, r) in enumerate(zip(specs, outs, ref_s)):
+        assert torch.equal(o, r), f"job {i}: dtype={x.dtype} bits={b} B={B} n={x.numel()}"
+    for i, (v, r) in enumerate(zip(vals, ref_v)):
+        assert torch.equal(v.view(torch.uint8), r.view(torch.uint8)), f"decompress job {i}"
+    plan.close()
+    dplan.close()
+
+
+def test_graph_replays_with_eager_work_between(fz):
+    fz_mod, batch = fz
+    specs = make_arrays(2, count=30)
+    ref_s, _ = plain_outputs(fz_mod, specs)
+    outs = [torch.empty_like(s) for s in ref_s]
+    plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
+    assert plan.config["use_graph"] == 1
+    for rep in range(4):
+        for o in outs:
+            o.fill_(rep + 1)
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        for i, (o, r) in enumerate(zip(outs, ref_s)):
+            assert torch.equal(o, r), f"replay {rep} job {i}"
+        del junk
+
+
+def test_autotune_keeps_bytes(fz):
+    fz_mod, batch = fz
+    specs = make_arrays(3, count=40)
+    ref_s, _ = plain_outputs(fz_mod, specs)
+    outs = [torch.empty_like(s) for s in ref_s]
+    plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
+    chosen = plan.autotune(reps=1)
+    assert chosen["values_per_thread"] in (4, 8) and chosen["workgroups"] > 0
+    for o in outs:
+        o.zero_()
+    plan.launch()
+    torch.cuda.synchronize()
+    for o, r in zip(outs, ref_s):
+        assert torch.equal(o, r)
+
+
+def test_batch_rejects_bad_jobs(fz):
+    fz_mod, batch = fz
+    x = torch.ones(100, device="cuda")
+    small = torch.empty(10, dtype=torch.uint8, device="cuda")
+    with pytest.raises(fz_mod.Frsz2Error) as e:
+        batch.BatchPlan([batch.compress_job(x, small, 16, 32)])
+    assert e.value.code == 3
+    out = torch.empty(fz_mod.compressed_size(torch.float32, 33, 32, 100), dtype=torch.uint8, device="cuda")
+    with pytest.raises(fz_mod.Frsz2Error) as e:
+        batch.BatchPlan([batch.compress_job(x, out, 33, 32)])
+    assert e.value.code == 2 and "Unsupported number of bits: 33" in str(e.value)
