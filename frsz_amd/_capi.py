@@ -88,11 +88,18 @@ def version() -> str:
     return lib.frsz2_version().decode()
 
 
-def tune_set(op: int, dtype: int, bits: int, values_per_thread: int, nontemporal: bool) -> None:
-    check(lib.frsz2_tune_set(op, dtype, bits, values_per_thread, int(bool(nontemporal))))
+TUNE_NONTEMPORAL = 1
+TUNE_STAGED_STORES = 2
+
+
+def tune_set(op: int, dtype: int, bits: int, values_per_thread: int, nontemporal: bool = True,
+             staged_stores: bool = False) -> None:
+    flags = (TUNE_NONTEMPORAL if nontemporal else 0) | (TUNE_STAGED_STORES if staged_stores else 0)
+    check(lib.frsz2_tune_set(op, dtype, bits, values_per_thread, flags))
 
 
 def tune_get(op: int, dtype: int, bits: int):
-    v, nt = ctypes.c_int(), ctypes.c_int()
-    check(lib.frsz2_tune_get(op, dtype, bits, ctypes.byref(v), ctypes.byref(nt)))
-    return v.value, bool(nt.value)
+    """(values_per_thread, nontemporal, staged_stores) of (op, dtype, bits)."""
+    v, fl = ctypes.c_int(), ctypes.c_int()
+    check(lib.frsz2_tune_get(op, dtype, bits, ctypes.byref(v), ctypes.byref(fl)))
+    return v.value, bool(fl.value & TUNE_NONTEMPORAL), bool(fl.value & TUNE_STAGED_STORES)

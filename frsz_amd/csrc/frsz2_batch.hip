@@ -83,7 +83,8 @@ run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred
     tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);                                     \
   break
 #define FRSZ2_D(T, U, PK)                                                                                    \
-  tile::decompress_tile<T, U, FRSZ2_V(T), PK, true>(i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds); \
+  tile::decompress_tile<T, U, FRSZ2_V(T), PK, true, false>(i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds, \
+                                                          nullptr);                                           \
   break
   switch (J.cls) {
     case 0: FRSZ2_C(float, 8, false);

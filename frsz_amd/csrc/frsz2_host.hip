@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 #include <cstdint>
 #include <string>
 
@@ -21,10 +22,10 @@ namespace {
 thread_local std::string g_error;
 
 // tuning table of the tiled kernels: [op][dtype][U = 8,16,32,64][bits < U],
-// each entry vpt * 2 + nt.  Defaults come from measurements on MI355X;
+// each entry vpt * 4 + stage * 2 + nt.  Defaults come from measurements on MI355X;
 // frsz2_tune_set (the autotuner) overrides them per process.
 std::atomic<int> g_variant[2][2][4][2];
-std::atomic<bool> g_variant_init{ false };
+std::once_flag g_variant_once;
 
 int
 u_index(int bits)
@@ -33,18 +34,53 @@ u_index(int bits)
   return U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3;
 }
 
-void
-init_variants()
+// encoded table entry
+constexpr int
+venc(int vpt, int stage, int nt)
 {
-  if (g_variant_init.load(std::memory_order_acquire)) {
-    return;
-  }
+  return vpt * 4 + stage * 2 + nt;
+}
+
+// Defaults measured on MI355X with the plain kernels (1 GiB float64 field,
+// 10^8-value float32 arrays): [op][dtype][U index][packed].
+void
+fill_variants()
+{
+  static const int table[2][2][4][2] = {
+    // compress
+    { // float32: U8, U16, U32 (U64 unused)
+      { { venc(8, 0, 0), venc(8, 0, 0) },
+        { venc(8, 0, 1), venc(8, 0, 0) },
+        { venc(4, 0, 1), venc(8, 0, 0) },
+        { venc(4, 0, 1), venc(4, 0, 1) } },
+      // float64
+      { { venc(4, 0, 1), venc(4, 0, 1) },
+        { venc(4, 0, 1), venc(4, 0, 1) },
+        { venc(4, 0, 1), venc(4, 0, 1) },
+        { venc(4, 0, 1), venc(4, 0, 1) } } },
+    // decompress
+    { // float32
+      { { venc(16, 1, 1), venc(8, 0, 1) },
+        { venc(16, 1, 1), venc(8, 0, 1) },
+        { venc(8, 0, 1), venc(8, 0, 1) },
+        { venc(4, 0, 1), venc(4, 0, 1) } },
+      // float64
+      { { venc(8, 1, 1), venc(4, 0, 1) },
+        { venc(8, 1, 1), venc(4, 0, 1) },
+        { venc(8, 1, 1), venc(4, 0, 1) },
+        { venc(8, 1, 1), venc(4, 0, 1) } } }
+  };
   for (int op = 0; op < 2; ++op)
     for (int d = 0; d < 2; ++d)
       for (int u = 0; u < 4; ++u)
         for (int pk = 0; pk < 2; ++pk)
-          g_variant[op][d][u][pk].store(4 * 2 + 1, std::memory_order_relaxed);
-  g_variant_init.store(true, std::memory_order_release);
+          g_variant[op][d][u][pk].store(table[op][d][u][pk], std::memory_order_relaxed);
+}
+
+void
+init_variants()
+{
+  std::call_once(g_variant_once, fill_variants);
 }
 } // namespace
 
@@ -53,15 +89,15 @@ get_variant(int op, int dtype, int bits)
 {
   init_variants();
   const int v = g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].load(std::memory_order_relaxed);
-  return variant{ v / 2, v % 2 };
+  return variant{ v / 4, v % 2, (v / 2) % 2 };
 }
 
 int
-set_variant(int op, int dtype, int bits, int vpt, int nt)
+set_variant(int op, int dtype, int bits, int vpt, int nt, int stage)
 {
   init_variants();
-  g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].store(vpt * 2 + (nt ? 1 : 0),
-                                                                             std::memory_order_relaxed);
+  g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].store(
+    vpt * 4 + (stage ? 2 : 0) + (nt ? 1 : 0), std::memory_order_relaxed);
   return 0;
 }
 
@@ -120,7 +156,7 @@ frsz2_last_error(void)
 }
 
 int
-frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int nontemporal)
+frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags)
 {
   using namespace frsz2;
   clear_error();
@@ -132,11 +168,12 @@ frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int nontempor
       (values_per_thread != 4 && values_per_thread != 8 && values_per_thread != 16)) {
     return set_error(FRSZ2_EARG, "op must be 0/1 and values_per_thread 4, 8 or 16");
   }
-  return set_variant(op, dtype, bits, values_per_thread, nontemporal);
+  return set_variant(op, dtype, bits, values_per_thread, flags & FRSZ2_TUNE_NONTEMPORAL,
+                     op == OP_DECOMPRESS ? (flags & FRSZ2_TUNE_STAGED_STORES) : 0);
 }
 
 int
-frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* nontemporal)
+frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags)
 {
   using namespace frsz2;
   clear_error();
@@ -144,12 +181,12 @@ frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* nontemp
   if (rc) {
     return rc;
   }
-  if ((op != OP_COMPRESS && op != OP_DECOMPRESS) || !values_per_thread || !nontemporal) {
+  if ((op != OP_COMPRESS && op != OP_DECOMPRESS) || !values_per_thread || !flags) {
     return set_error(FRSZ2_EARG, "bad op or null pointer");
   }
   const variant v = get_variant(op, dtype, bits);
   *values_per_thread = v.vpt;
-  *nontemporal = v.nt;
+  *flags = (v.nt ? FRSZ2_TUNE_NONTEMPORAL : 0) | (v.stage ? FRSZ2_TUNE_STAGED_STORES : 0);
   return FRSZ2_OK;
 }
 

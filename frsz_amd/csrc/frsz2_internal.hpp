@@ -14,8 +14,9 @@ void clear_error();
 
 int check_params(int dtype, int bits, std::uint64_t block);
 
-// kernel variant of the tiled plain kernels: values per thread (4, 8, 16)
-// and whether global accesses carry the non-temporal hint
+// kernel variant of the tiled plain kernels: values per thread (4, 8, 16),
+// whether global accesses carry the non-temporal hint, and (decompress)
+// whether the output goes out through an LDS staging buffer
 enum
 {
   OP_COMPRESS = 0,
@@ -25,9 +26,10 @@ struct variant
 {
   int vpt;
   int nt;
+  int stage;
 };
 variant get_variant(int op, int dtype, int bits);
-int set_variant(int op, int dtype, int bits, int vpt, int nt);
+int set_variant(int op, int dtype, int bits, int vpt, int nt, int stage);
 int hip_status(hipError_t e, const char* what);
 
 int compress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t n, void* d_out,

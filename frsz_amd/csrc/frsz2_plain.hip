@@ -44,15 +44,16 @@ __global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw*
   }
 }
 
-template<typename T, int U, int VPT, bool PACKED, bool NT>
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE>
 __global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __restrict__ in,
                                                         typename fmt<T>::uw* __restrict__ out,
                                                         tplan p,
                                                         std::uint32_t ntiles)
 {
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
+  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT>() : 1];
   const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
-  tile::decompress_tile<T, U, VPT, PACKED, NT>(in, out, p, t, lds);
+  tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE>(in, out, p, t, lds, stage);
 }
 
 // One thread per exponent block, any B; builds each payload word from the
@@ -172,7 +173,7 @@ launch_compress_tiled(const void* in, void* out, const layout& L, int lb, hipStr
   return hipGetLastError();
 }
 
-template<typename T, int U, int VPT, bool NT>
+template<typename T, int U, int VPT, bool NT, bool STAGE>
 hipError_t
 launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
 {
@@ -185,10 +186,10 @@ launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipS
   const auto* x = static_cast<const std::uint8_t*>(in);
   auto* o = static_cast<typename fmt<T>::uw*>(out);
   if (L.bits == U) {
-    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false, NT, STAGE>), dim3(nt), dim3(WG), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt));
   } else {
-    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true, NT, STAGE>), dim3(nt), dim3(WG), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt));
   }
   return hipGetLastError();
@@ -261,20 +262,22 @@ run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, h
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
-    switch (v.vpt * 2 + v.nt) {
-      case 8:
-        return launch_decompress_tiled<T, U, 4, false>(in, out, L, lb, s);
-      case 9:
-        return launch_decompress_tiled<T, U, 4, true>(in, out, L, lb, s);
-      case 16:
-        return launch_decompress_tiled<T, U, 8, false>(in, out, L, lb, s);
-      case 17:
-        return launch_decompress_tiled<T, U, 8, true>(in, out, L, lb, s);
-      case 32:
-        return launch_decompress_tiled<T, U, 16, false>(in, out, L, lb, s);
-      default:
-        return launch_decompress_tiled<T, U, 16, true>(in, out, L, lb, s);
+#define FRSZ2_DL(V, N, S) return launch_decompress_tiled<T, U, V, N, S>(in, out, L, lb, s)
+    switch (v.vpt * 4 + v.stage * 2 + v.nt) {
+      case 16: FRSZ2_DL(4, false, false);
+      case 17: FRSZ2_DL(4, true, false);
+      case 18: FRSZ2_DL(4, false, true);
+      case 19: FRSZ2_DL(4, true, true);
+      case 32: FRSZ2_DL(8, false, false);
+      case 33: FRSZ2_DL(8, true, false);
+      case 34: FRSZ2_DL(8, false, true);
+      case 35: FRSZ2_DL(8, true, true);
+      case 64: FRSZ2_DL(16, false, false);
+      case 65: FRSZ2_DL(16, true, false);
+      case 66: FRSZ2_DL(16, false, true);
+      default: FRSZ2_DL(16, true, true);
     }
+#undef FRSZ2_DL
   }
   return launch_decompress_generic<T, U>(in, out, L, s);
 }

@@ -479,14 +479,27 @@ load_bytes(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[NB >= 4 ? NB
 // chunks, each load / store instruction of the workgroup covers one
 // contiguous span.  With codes staged in LDS (bits < U) a thread has one
 // chunk of VPT consecutive values.  All loads of a thread are issued before
-// any value is decoded.
-template<typename T, int U, int VPT, bool PACKED, bool NT>
+// any value is decoded.  With STAGE the decoded tile goes through LDS and
+// out with 16 bytes per lane per store instruction, so every instruction
+// writes whole contiguous cache lines (a lane's own values are 32+ bytes, so
+// storing them directly leaves each instruction half-filling its lines, and
+// L2 then writes some lines back twice).
+// LDS dwords of the output staging buffer of a decompress tile
+template<typename T, int VPT>
+constexpr int
+stage_dwords()
+{
+  return WG * VPT * static_cast<int>(sizeof(T)) / 4;
+}
+
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE>
 __device__ __forceinline__ void
 decompress_tile(const std::uint8_t* __restrict__ in,
                 typename fmt<T>::uw* __restrict__ out,
                 const tplan& p,
                 std::uint64_t tile,
-                std::uint32_t* lds)
+                std::uint32_t* lds,
+                std::uint32_t* stage)
 {
   using uw = typename fmt<T>::uw;
   constexpr int TILE = WG * VPT;
@@ -568,21 +581,54 @@ decompress_tile(const std::uint8_t* __restrict__ in,
       }
     }
   }
+  if constexpr (STAGE) {
+    uw* sv = reinterpret_cast<uw*>(stage);
 #pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WG + t) * VC);
-    uw y[VC];
-#pragma unroll
-    for (int v = 0; v < VC; ++v) {
-      y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
-    }
-    if (i0 + VC <= p.n) {
-      store_values<T, VC, NT>(out + i0, y);
-    } else {
+    for (int c = 0; c < NCH; ++c) {
+      const std::uint32_t local0 = static_cast<std::uint32_t>((c * WG + t) * VC);
+      uw y[VC];
 #pragma unroll
       for (int v = 0; v < VC; ++v) {
-        if (i0 + v < p.n) {
-          out[i0 + v] = y[v];
+        y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
+      }
+      store_values<T, VC, false>(sv + local0, y);
+    }
+    __syncthreads();
+    constexpr int CPV = 16 / static_cast<int>(sizeof(T));
+    constexpr int NQ = TILE / CPV; // 16-byte chunks of the tile
+    const std::uint64_t left = p.n - base;
+    const std::uint32_t nvals = left < static_cast<std::uint64_t>(TILE) ? static_cast<std::uint32_t>(left) : TILE;
+    const u32x4* s4 = reinterpret_cast<const u32x4*>(stage);
+    u32x4* o4 = reinterpret_cast<u32x4*>(out + base);
+#pragma unroll
+    for (int q0 = 0; q0 < NQ; q0 += WG) {
+      const std::uint32_t q = static_cast<std::uint32_t>(q0 + t);
+      if ((q + 1) * CPV <= nvals) {
+        st<NT>(s4[q], o4 + q);
+      } else if (q * CPV < nvals) {
+        for (std::uint32_t v = q * CPV; v < nvals; ++v) {
+          out[base + v] = sv[v];
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WG + t) * VC);
+      uw y[VC];
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
+      }
+      if (i0 + VC <= p.n) {
+        store_values<T, VC, NT>(out + i0, y);
+      } else {
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          if (i0 + v < p.n) {
+            out[i0 + v] = y[v];
+          }
         }
       }
     }

@@ -119,12 +119,18 @@ int frsz2_gather(int dtype,
 /* Kernel variant of the tiled plain kernels for (op, dtype, bits): op 0 =
  * compress, 1 = decompress; values_per_thread 4, 8 or 16 (a tile is 256 x
  * that many values, so it also bounds the largest B the tiled path takes);
- * nontemporal 0/1 puts the non-temporal hint on the streaming accesses.
- * Process-wide; the batch autotuner sets it.  No counterpart in the
- * reference (its kernels have a fixed launch shape,
+ * flags: FRSZ2_TUNE_NONTEMPORAL puts the non-temporal hint on the streaming
+ * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) writes each tile's output
+ * through LDS with whole-cache-line stores.  Process-wide.  No counterpart
+ * in the reference (its kernels have a fixed launch shape,
  * test/test_frsz_gpu.cu:325-330). */
-int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int nontemporal);
-int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* nontemporal);
+enum
+{
+  FRSZ2_TUNE_NONTEMPORAL = 1,
+  FRSZ2_TUNE_STAGED_STORES = 2
+};
+int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags);
+int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags);
 
 /* ---- batched: many arrays in one persistent, graph-captured launch ----
  * No counterpart in the reference: its kernels take one array per launch

@@ -118,7 +118,7 @@ def test_batch_rejects_bad_jobs(fz):
     with pytest.raises(fz_mod.Frsz2Error) as e:
         batch.BatchPlan([batch.compress_job(x, small, 16, 32)])
     assert e.value.code == 3
-    out = torch.empty(fz_mod.compressed_size(torch.float32, 33, 32, 100), dtype=torch.uint8, device="cuda")
+    out = torch.empty(fz_mod.compressed_size(torch.float64, 33, 32, 100), dtype=torch.uint8, device="cuda")
     with pytest.raises(fz_mod.Frsz2Error) as e:
         batch.BatchPlan([batch.compress_job(x, out, 33, 32)])
     assert e.value.code == 2 and "Unsupported number of bits: 33" in str(e.value)
