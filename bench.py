@@ -46,10 +46,12 @@ def parse():
     return ap.parse_args()
 
 
-def bench_batched(count, steps, seed=4):
+def bench_batched(count, steps, seed=4, target_input_bytes=None):
     """4096 arrays, lengths log-uniform over [1e3, 1.6e7], 60% float32 /
     40% float64, bits from {16, 32}, B = 32: one autotuned persistent
-    graph launch vs a C++ loop of plain per-array launches, per direction."""
+    graph launch vs a C++ loop of plain per-array launches, per direction.
+    With target_input_bytes every length is divided by one factor so the
+    input totals that many bytes (the workload's stated "about 4 GiB")."""
     import numpy as np
     import torch
 
@@ -58,6 +60,11 @@ def bench_batched(count, steps, seed=4):
     from frsz_amd.workloads import batched_lengths, smooth_signal_torch
 
     lengths, is_f64, bits = batched_lengths(count, seed=seed)
+    scale = 1.0
+    if target_input_bytes:
+        full = float(np.sum(lengths * np.where(is_f64, 8, 4)))
+        scale = full / target_input_bytes
+        lengths = np.maximum(1, np.round(lengths / scale)).astype(np.int64)
     B = 32
     xs, streams, outs = [], [], []
     alg = 0
@@ -85,7 +92,8 @@ def bench_batched(count, steps, seed=4):
         return e0.elapsed_time(e1) / k * 1e-3
 
     res = {"arrays": count, "total_values": int(lengths.sum()), "float64_share": round(float(is_f64.mean()), 3),
-           "algorithmic_bytes_per_direction": int(alg), "block": B}
+           "algorithmic_bytes_per_direction": int(alg), "block": B,
+           "length_scale": round(scale, 3), "min_len": int(lengths.min()), "max_len": int(lengths.max())}
     # plain per-array launches from a C++ loop (the baseline), and the same captured in a graph
     plain_c = batch.BatchPlan(cjobs, {"plain_launches": 1, "use_graph": 0})
     plain_d = batch.BatchPlan(djobs, {"plain_launches": 1, "use_graph": 0})
@@ -298,6 +306,8 @@ def main():
     if not args.no_batched:
         print("bench: batched workload", file=sys.stderr, flush=True)
         line["batched"] = bench_batched(args.batched_arrays, args.batched_steps)
+        print("bench: batched workload scaled to 4 GiB of input", file=sys.stderr, flush=True)
+        line["batched_4gib"] = bench_batched(args.batched_arrays, args.batched_steps, target_input_bytes=1 << 32)
     if not args.no_cpu:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(x_host, bits, block, min(args.cpu_sample, n))

@@ -41,7 +41,7 @@ class Job(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     _fields_ = [
         ("workgroups", ctypes.c_int),
-        ("values_per_thread", ctypes.c_int),
+        ("tile_shape", ctypes.c_int),
         ("tiles_per_claim", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("plain_launches", ctypes.c_int),
@@ -86,7 +86,7 @@ class BatchPlan:
         handle = ctypes.c_void_p()
         cfg = None
         if config is not None:
-            cfg = BatchConfig(config.get("workgroups", 0), config.get("values_per_thread", 8),
+            cfg = BatchConfig(config.get("workgroups", 0), config.get("tile_shape", 1),
                               config.get("tiles_per_claim", 4), config.get("use_graph", 1),
                               config.get("plain_launches", 0))
         check(_L.frsz2_batch_create(arr, len(jobs), ctypes.byref(cfg) if cfg is not None else None,
@@ -112,7 +112,7 @@ class BatchPlan:
     def config(self, cfg: dict) -> None:
         cur = self.config
         cur.update(cfg)
-        c = BatchConfig(cur["workgroups"], cur["values_per_thread"], cur["tiles_per_claim"], cur["use_graph"],
+        c = BatchConfig(cur["workgroups"], cur["tile_shape"], cur["tiles_per_claim"], cur["use_graph"],
                         cur["plain_launches"])
         check(_L.frsz2_batch_set_config(self._h, ctypes.byref(c)))
 

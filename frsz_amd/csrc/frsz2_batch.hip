@@ -13,8 +13,9 @@
 // launches inside the same graph.  The counter is reset by a one-thread
 // kernel captured at the head of the graph.
 //
-// The autotuner times candidate (workgroups, values per thread, chunk)
-// settings on the plan's own jobs and keeps the fastest.
+// Tile shapes are per class and come in three settings (narrow, balanced,
+// wide; see class_vpt).  The autotuner times candidate (workgroups, tile
+// shape, chunk) settings on the plan's own jobs and keeps the fastest.
 //
 // No counterpart in the reference: its kernels take one array per launch
 // (include/frsz2.hpp:1258-1288).
@@ -67,58 +68,117 @@ class_of(int op, int dtype, int bits)
   return op * 14 + slot;
 }
 
-// V32 / V64: values per thread of float32 / float64 tiles
-template<int V32, int V64>
+// Per-class tile shape, three settings:
+//   0 narrow:   4 values per thread for every class (smallest tiles, finest
+//               load balance, fewest registers)
+//   1 balanced: up to 8 values per thread, float64 decompress through the
+//               LDS output stage
+//   2 wide:     the shapes measured fastest for the plain kernels (up to 16
+//               values per thread, staged decompress stores)
+// In one kernel every class shares the register allocation of the widest,
+// so wider shapes trade occupancy for bytes in flight; the autotuner picks.
+__host__ __device__ constexpr int
+class_vpt(int cls, int shape)
+{
+  const int op = cls / 14;
+  const int slot = cls % 14;
+  const bool f32 = slot < 6;
+  const int ui = f32 ? slot / 2 : (slot - 6) / 2;
+  const bool packed = (slot % 2) == 1;
+  if (shape == 0) {
+    return 4;
+  }
+  if (op == 0) {
+    return f32 ? ((ui == 2 && !packed) ? 4 : 8) : 4;
+  }
+  if (shape == 1) {
+    return f32 ? 8 : 4;
+  }
+  if (f32) {
+    return packed ? 8 : (ui == 2 ? 8 : 16);
+  }
+  return packed ? 4 : 8;
+}
+
+__host__ __device__ constexpr bool
+class_stage(int cls, int shape)
+{
+  const int slot = cls % 14;
+  const bool f32 = slot < 6;
+  const bool packed = (slot % 2) == 1;
+  if (shape == 0 || cls / 14 != 1 || packed) {
+    return false;
+  }
+  if (shape == 1) {
+    return !f32;
+  }
+  return !(f32 && slot / 2 == 2);
+}
+
+// LDS dwords of the batch kernel: the largest payload image or output stage
+// of any class (a tile uses one or the other)
+template<int SHAPE>
+constexpr int
+batch_lds_dwords()
+{
+  return SHAPE == 0 ? tile::lds_dwords<4>() : tile::lds_dwords<8>();
+}
+static_assert(tile::stage_dwords<float, 16>() <= tile::lds_dwords<8>(), "stage fits");
+static_assert(tile::stage_dwords<double, 8>() <= tile::lds_dwords<8>(), "stage fits");
+
+template<int SHAPE>
 __device__ __forceinline__ void
 run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred)
 {
   const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb, 0 };
   auto* o8 = static_cast<std::uint8_t*>(J.out);
   const auto* i8 = static_cast<const std::uint8_t*>(J.in);
-#define FRSZ2_V(T) (sizeof(T) == 4 ? V32 : V64)
-#define FRSZ2_C(T, U, PK)                                                                                    \
-  tile::compress_tile<T, U, FRSZ2_V(T), PK, true>(                                                           \
-    static_cast<const typename fmt<T>::uw*>(J.in), o8, p, t, lds, wred);                                     \
-  if (t == 0)                                                                                                \
-    tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);                                     \
-  break
-#define FRSZ2_D(T, U, PK)                                                                                    \
-  tile::decompress_tile<T, U, FRSZ2_V(T), PK, true, false>(i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds, \
-                                                          nullptr);                                           \
-  break
+#define FRSZ2_C(K, T, U, PK)                                                                                 \
+  case K:                                                                                                    \
+    tile::compress_tile<T, U, class_vpt(K, SHAPE), PK, true>(static_cast<const typename fmt<T>::uw*>(J.in),   \
+                                                            o8, p, t, lds, wred);                            \
+    if (t == 0)                                                                                              \
+      tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);                                   \
+    break
+#define FRSZ2_D(K, T, U, PK)                                                                                 \
+  case K:                                                                                                    \
+    tile::decompress_tile<T, U, class_vpt(K, SHAPE), PK, true, class_stage(K, SHAPE)>(                         \
+      i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds, lds);                                         \
+    break
   switch (J.cls) {
-    case 0: FRSZ2_C(float, 8, false);
-    case 1: FRSZ2_C(float, 8, true);
-    case 2: FRSZ2_C(float, 16, false);
-    case 3: FRSZ2_C(float, 16, true);
-    case 4: FRSZ2_C(float, 32, false);
-    case 5: FRSZ2_C(float, 32, true);
-    case 6: FRSZ2_C(double, 8, false);
-    case 7: FRSZ2_C(double, 8, true);
-    case 8: FRSZ2_C(double, 16, false);
-    case 9: FRSZ2_C(double, 16, true);
-    case 10: FRSZ2_C(double, 32, false);
-    case 11: FRSZ2_C(double, 32, true);
-    case 12: FRSZ2_C(double, 64, false);
-    case 13: FRSZ2_C(double, 64, true);
-    case 14: FRSZ2_D(float, 8, false);
-    case 15: FRSZ2_D(float, 8, true);
-    case 16: FRSZ2_D(float, 16, false);
-    case 17: FRSZ2_D(float, 16, true);
-    case 18: FRSZ2_D(float, 32, false);
-    case 19: FRSZ2_D(float, 32, true);
-    case 20: FRSZ2_D(double, 8, false);
-    case 21: FRSZ2_D(double, 8, true);
-    case 22: FRSZ2_D(double, 16, false);
-    case 23: FRSZ2_D(double, 16, true);
-    case 24: FRSZ2_D(double, 32, false);
-    case 25: FRSZ2_D(double, 32, true);
-    case 26: FRSZ2_D(double, 64, false);
-    default: FRSZ2_D(double, 64, true);
+    FRSZ2_C(0, float, 8, false);
+    FRSZ2_C(1, float, 8, true);
+    FRSZ2_C(2, float, 16, false);
+    FRSZ2_C(3, float, 16, true);
+    FRSZ2_C(4, float, 32, false);
+    FRSZ2_C(5, float, 32, true);
+    FRSZ2_C(6, double, 8, false);
+    FRSZ2_C(7, double, 8, true);
+    FRSZ2_C(8, double, 16, false);
+    FRSZ2_C(9, double, 16, true);
+    FRSZ2_C(10, double, 32, false);
+    FRSZ2_C(11, double, 32, true);
+    FRSZ2_C(12, double, 64, false);
+    FRSZ2_C(13, double, 64, true);
+    FRSZ2_D(14, float, 8, false);
+    FRSZ2_D(15, float, 8, true);
+    FRSZ2_D(16, float, 16, false);
+    FRSZ2_D(17, float, 16, true);
+    FRSZ2_D(18, float, 32, false);
+    FRSZ2_D(19, float, 32, true);
+    FRSZ2_D(20, double, 8, false);
+    FRSZ2_D(21, double, 8, true);
+    FRSZ2_D(22, double, 16, false);
+    FRSZ2_D(23, double, 16, true);
+    FRSZ2_D(24, double, 32, false);
+    FRSZ2_D(25, double, 32, true);
+    FRSZ2_D(26, double, 64, false);
+    FRSZ2_D(27, double, 64, true);
+    default:
+      break;
   }
 #undef FRSZ2_C
 #undef FRSZ2_D
-#undef FRSZ2_V
 }
 
 // job owning global tile g, searching from hint (jobs sorted by tile0)
@@ -141,14 +201,14 @@ find_job(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, int hint)
   return lo;
 }
 
-template<int V32, int V64>
+template<int SHAPE>
 __global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
                                               int njobs,
                                               std::uint64_t total_tiles,
                                               unsigned long long* counter,
                                               int chunk)
 {
-  __shared__ std::uint32_t lds[tile::lds_dwords<(V32 > V64 ? V32 : V64)>()];
+  __shared__ std::uint32_t lds[batch_lds_dwords<SHAPE>()];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
   __shared__ unsigned long long claim;
   int j = 0;
@@ -178,7 +238,7 @@ __global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
     for (std::uint64_t g = g0; g < g1; ++g) {
       j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, j));
       const bjob J = jobs[j];
-      run_tile<V32, V64>(J, g - J.tile0, lds, wred);
+      run_tile<SHAPE>(J, g - J.tile0, lds, wred);
     }
   }
 }
@@ -257,10 +317,7 @@ validate_job(const frsz2_job& j, int idx)
   return FRSZ2_OK;
 }
 
-// float64 tiles always use this many values per thread (measured best)
-constexpr int V64 = 4;
-
-// (re)build the device descriptors for cfg.values_per_thread (float32)
+// (re)build the device descriptors for cfg.tile_shape
 int
 build_descriptors(frsz2_batch_plan* P)
 {
@@ -269,8 +326,8 @@ build_descriptors(frsz2_batch_plan* P)
   std::uint64_t tiles = 0;
   for (std::size_t i = 0; i < P->jobs.size(); ++i) {
     const frsz2_job& j = P->jobs[i];
-    const std::uint64_t TILE =
-      static_cast<std::uint64_t>(WG) * (j.dtype == FRSZ2_FLOAT32 ? P->cfg.values_per_thread : V64);
+    const int cls = class_of(j.op, j.dtype, j.bits);
+    const std::uint64_t TILE = static_cast<std::uint64_t>(WG) * class_vpt(cls, P->cfg.tile_shape);
     const int lb = ilog2_exact(j.block);
     if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
       continue;
@@ -294,7 +351,7 @@ build_descriptors(frsz2_batch_plan* P)
     d.CB = static_cast<std::uint32_t>(L.CB);
     d.bits = j.bits;
     d.lb = lb;
-    d.cls = class_of(j.op, j.dtype, j.bits);
+    d.cls = cls;
     P->host_desc.push_back(d);
     tiles += nt;
   }
@@ -341,12 +398,20 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
     if (chunk > 0) {
       hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, P->d_counter);
     }
-    if (P->cfg.values_per_thread == 8) {
-      hipLaunchKernelGGL((k_batch<8, V64>), dim3(static_cast<unsigned>(grid)), dim3(WG), 0, s, P->d_desc, P->ndesc,
-                         P->total_tiles, P->d_counter, chunk);
-    } else {
-      hipLaunchKernelGGL((k_batch<4, V64>), dim3(static_cast<unsigned>(grid)), dim3(WG), 0, s, P->d_desc, P->ndesc,
-                         P->total_tiles, P->d_counter, chunk);
+    const dim3 gd(static_cast<unsigned>(grid));
+    switch (P->cfg.tile_shape) {
+      case 0:
+        hipLaunchKernelGGL((k_batch<0>), gd, dim3(WG), 0, s, P->d_desc, P->ndesc, P->total_tiles, P->d_counter,
+                           chunk);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_batch<1>), gd, dim3(WG), 0, s, P->d_desc, P->ndesc, P->total_tiles, P->d_counter,
+                           chunk);
+        break;
+      default:
+        hipLaunchKernelGGL((k_batch<2>), gd, dim3(WG), 0, s, P->d_desc, P->ndesc, P->total_tiles, P->d_counter,
+                           chunk);
+        break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -421,8 +486,8 @@ int
 apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
 {
   frsz2_batch_config n = c;
-  if (n.values_per_thread != 4 && n.values_per_thread != 8) {
-    n.values_per_thread = 8;
+  if (n.tile_shape < 0 || n.tile_shape > 2) {
+    n.tile_shape = 1;
   }
   if (n.workgroups <= 0) {
     n.workgroups = 4 * P->cus;
@@ -430,7 +495,7 @@ apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
   if (n.tiles_per_claim < 0) {
     n.tiles_per_claim = 0;
   }
-  const bool rebuild = n.values_per_thread != P->cfg.values_per_thread || P->host_desc.empty();
+  const bool rebuild = n.tile_shape != P->cfg.tile_shape || P->host_desc.empty();
   P->cfg = n;
   if (rebuild) {
     const int rc = build_descriptors(P);
@@ -493,12 +558,12 @@ frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config
     c = *cfg;
   } else {
     c.workgroups = 0;
-    c.values_per_thread = 8;
+    c.tile_shape = 1;
     c.tiles_per_claim = 4;
     c.use_graph = 1;
     c.plain_launches = 0;
   }
-  P->cfg.values_per_thread = 0; // force a descriptor build
+  P->cfg.tile_shape = -1; // force a descriptor build
   const int rc = apply_config(P, c);
   if (rc) {
     destroy(P);
@@ -564,15 +629,15 @@ frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, fr
   }
   frsz2_batch_config best = plan->cfg;
   float best_ms = -1.0f;
-  const int vpts[2] = { 4, 8 };
+  const int shapes[3] = { 0, 1, 2 };
   const int wg_per_cu[4] = { 2, 4, 8, 16 };
   const int chunks[4] = { 0, 1, 4, 16 };
   int rc = FRSZ2_OK;
-  for (int vi = 0; vi < 2 && rc == FRSZ2_OK; ++vi) {
+  for (int vi = 0; vi < 3 && rc == FRSZ2_OK; ++vi) {
     for (int wi = 0; wi < 4 && rc == FRSZ2_OK; ++wi) {
       for (int ci = 0; ci < 4 && rc == FRSZ2_OK; ++ci) {
         frsz2_batch_config c{};
-        c.values_per_thread = vpts[vi];
+        c.tile_shape = shapes[vi];
         c.workgroups = wg_per_cu[wi] * plan->cus;
         c.tiles_per_claim = chunks[ci];
         c.use_graph = 0;

@@ -160,7 +160,10 @@ typedef struct frsz2_job
 typedef struct frsz2_batch_config
 {
   int workgroups;        /* persistent grid size (0 = 4 per CU) */
-  int values_per_thread; /* 4 or 8 for float32 tiles (float64 tiles use 4) */
+  int tile_shape;        /* 0 narrow: 4 values per thread for every class;
+                            1 balanced: up to 8, float64 decompress staged;
+                            2 wide: the plain kernels' fastest shapes (up to
+                            16 values per thread, staged decompress stores) */
   int tiles_per_claim;   /* tiles per atomic claim; 0 = static round-robin */
   int use_graph;         /* 1 = launch through a captured hipGraph */
   int plain_launches;    /* 1 = no persistent kernel: one plain frsz2_compress /
@@ -171,7 +174,7 @@ typedef struct frsz2_batch_config
 typedef struct frsz2_batch_plan frsz2_batch_plan;
 
 /* Validate the jobs, upload their descriptors and capture the launch.
- * cfg NULL = defaults (4 workgroups per CU, 8 values per thread, 4 tiles per
+ * cfg NULL = defaults (4 workgroups per CU, balanced tiles, 4 tiles per
  * claim, graph on).  Jobs must not overlap in memory. */
 int frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config* cfg,
                        frsz2_batch_plan** plan);

@@ -50,9 +50,9 @@ def plain_outputs(fz_mod, specs):
 
 
 CONFIGS = [None,
-           {"workgroups": 64, "values_per_thread": 4, "tiles_per_claim": 0, "use_graph": 1},
-           {"workgroups": 1024, "values_per_thread": 8, "tiles_per_claim": 1, "use_graph": 0},
-           {"workgroups": 512, "values_per_thread": 8, "tiles_per_claim": 16, "use_graph": 1},
+           {"workgroups": 64, "tile_shape": 0, "tiles_per_claim": 0, "use_graph": 1},
+           {"workgroups": 1024, "tile_shape": 2, "tiles_per_claim": 1, "use_graph": 0},
+           {"workgroups": 512, "tile_shape": 2, "tiles_per_claim": 16, "use_graph": 1},
            {"plain_launches": 1, "use_graph": 1}]
 
 
@@ -102,7 +102,7 @@ def test_autotune_keeps_bytes(fz):
     outs = [torch.empty_like(s) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
     chosen = plan.autotune(reps=1)
-    assert chosen["values_per_thread"] in (4, 8) and chosen["workgroups"] > 0
+    assert chosen["tile_shape"] in (0, 1, 2) and chosen["workgroups"] > 0
     for o in outs:
         o.zero_()
     plan.launch()
