@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--batched-arrays", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--no-copy", action="store_true", help="skip the 1 GiB device-copy calibration")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=20000.0)
+    ap.add_argument("--serve-budget-gib", type=int, default=64)
     return ap.parse_args()
 
 
@@ -133,6 +137,134 @@ def bench_batched(count, steps, seed=4, target_input_bytes=None):
     del xs, streams, outs
     torch.cuda.empty_cache()
     return res
+
+
+def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499):
+    """Poisson arrivals at `rate` requests/s for `seconds`: 50% compress, 30%
+    full decompress, 20% random-access reads of 1-4096 values from resident
+    streams; sizes log-uniform 4 KiB-256 MiB, float32 / float64 (half each),
+    bits from {16, 32}, B = 32; HBM budget `budget_gib` GiB.  Open-loop replay
+    through the serving runtime; latency = completion - scheduled arrival.
+    Every `verify_every`-th request keeps its output, which is then compared
+    with the same request run alone through the single-array API."""
+    import numpy as np
+    import torch
+
+    import frsz_amd
+    from frsz_amd import serve
+    from frsz_amd.workloads import smooth_signal_torch
+
+    rng = np.random.default_rng(seed)
+    nreq = int(seconds * rate)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    kind = rng.choice(3, size=nreq, p=[0.5, 0.3, 0.2])
+    kind[0] = 0
+    f64 = rng.random(nreq) < 0.5
+    bits = np.where(rng.random(nreq) < 0.5, 16, 32)
+    nbytes = np.exp(rng.uniform(np.log(4096), np.log(256 << 20), nreq))
+    B = 32
+    src = {False: smooth_signal_torch((256 << 20) // 4, seed=77, dtype=torch.float32),
+           True: smooth_signal_torch((256 << 20) // 8, seed=78, dtype=torch.float64)}
+    trace = np.zeros(nreq, dtype=serve.REQUEST_DTYPE)
+    n = np.zeros(nreq, dtype=np.int64)
+    offs = np.zeros(nreq, dtype=np.int64)
+    source = np.zeros(nreq, dtype=np.int64)
+    recent = []  # indices of the latest compress requests
+    for i in range(nreq):
+        if kind[i] != 0 and not recent:
+            kind[i] = 0
+        if kind[i] == 0:
+            vs = 8 if f64[i] else 4
+            n[i] = max(1, int(nbytes[i]) // vs)
+            total = src[bool(f64[i])].numel()
+            offs[i] = int(rng.integers(0, (total - n[i]) // 4 + 1)) * 4
+            recent.append(i)
+            if len(recent) > 64:
+                recent.pop(0)
+        else:
+            s = recent[int(rng.integers(0, len(recent)))]
+            source[i] = s
+            n[i] = n[s]
+            f64[i] = f64[s]
+            bits[i] = bits[s]
+    m = np.where(kind == 2, rng.integers(1, 4097, nreq), 0)
+    # gather indices, generated on the GPU in one pass
+    gi = np.nonzero(kind == 2)[0]
+    idx_all = None
+    idx_off = np.zeros(nreq, dtype=np.int64)
+    if gi.size:
+        counts = torch.from_numpy(m[gi]).cuda()
+        lim = torch.from_numpy(n[gi]).cuda()
+        rep = torch.repeat_interleave(lim, counts)
+        idx_all = (torch.rand(rep.numel(), dtype=torch.float64, device="cuda") * rep).long()
+        idx_all = torch.minimum(idx_all, rep - 1)
+        del rep
+        idx_off[gi] = np.concatenate([[0], np.cumsum(m[gi])[:-1]])
+    keep = (np.arange(nreq) % verify_every) == 0
+    trace["op"] = kind
+    trace["dtype"] = f64.astype(np.int32)
+    trace["bits"] = bits
+    trace["flags"] = np.where(keep, 0, serve.REQ_DISCARD)
+    trace["block"] = B
+    trace["n"] = n
+    base32, base64 = src[False].data_ptr(), src[True].data_ptr()
+    trace["values"] = np.where(kind == 0, np.where(f64, base64 + offs * 8, base32 + offs * 4), 0)
+    trace["source"] = source
+    trace["idx"] = np.where(kind == 2, (idx_all.data_ptr() if idx_all is not None else 0) + idx_off * 8, 0)
+    trace["m"] = m
+    trace["arrival"] = arrival
+    torch.cuda.synchronize()
+    print(f"bench: serving {nreq} requests over {seconds:.0f} s", file=sys.stderr, flush=True)
+    srv = serve.Server(hbm_budget=budget_gib << 30, slots=4, tile_shape=0, tiles_per_claim=4)
+    res, tickets = srv.replay(trace)
+    stats = srv.stats()
+    lat = (res["t_done"] - res["t_arrival"]) * 1e3
+    ok = bool(np.all(res["status"] == 0))
+    vs = np.where(f64, 8, 4)
+    makespan = float(res["t_done"].max() - 0.0)
+    # bit-exact check of the kept sample against the request run alone
+    same = True
+    checked = 0
+    for i in np.nonzero(keep)[0]:
+        dt = torch.float64 if f64[i] else torch.float32
+        r = {k: res[i][k] for k in res.dtype.names}
+        if kind[i] == 0:
+            x = src[bool(f64[i])][offs[i]: offs[i] + n[i]]
+            want = frsz_amd.compress(x, int(bits[i]), B)
+            got = srv.output(r, torch.uint8, want.numel())
+        else:
+            s = int(source[i])
+            x = src[bool(f64[s])][offs[s]: offs[s] + n[s]]
+            st = frsz_amd.compress(x, int(bits[i]), B)
+            if kind[i] == 1:
+                want = frsz_amd.decompress(st, int(n[i]), dt, int(bits[i]), B)
+                got = srv.output(r, dt, int(n[i]))
+            else:
+                idx = idx_all[idx_off[i]: idx_off[i] + m[i]]
+                want = frsz_amd.gather(st, int(n[i]), dt, int(bits[i]), B, idx)
+                got = srv.output(r, dt, int(m[i]))
+        same &= torch.equal(got.view(torch.uint8), want.view(torch.uint8))
+        checked += 1
+        srv.release(int(tickets[i]))
+    srv.close()
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 4) if a.size else None
+
+    names = {0: "compress", 1: "decompress", 2: "gather"}
+    per = {names[k]: {"count": int((kind == k).sum()), "p50_ms": pct(lat[kind == k], 50),
+                      "p99_ms": pct(lat[kind == k], 99)} for k in (0, 1, 2)}
+    payload = float(np.sum(np.where(kind == 2, m * vs, n * vs)))
+    del src, idx_all
+    torch.cuda.empty_cache()
+    return {"requests": nreq, "rate_offered": rate, "seconds": seconds, "hbm_budget_gib": budget_gib,
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "p999_ms": pct(lat, 99.9), "max_ms": pct(lat, 100),
+            "per_op": per, "throughput_req_s": round(nreq / makespan, 1),
+            "payload_gbs": round(payload / makespan / 1e9, 2), "all_ok": ok,
+            "bit_exact_sample": {"checked": checked, "same": bool(same)},
+            "launches": stats["launches"], "mean_requests_per_launch":
+                round(stats["launched_requests"] / max(stats["launches"], 1), 2),
+            "pool_peak_gib": round(stats["pool_peak"] / 2**30, 3), "max_queue": stats["max_queue"]}
 
 
 def cpu_baseline(x_host, bits, block, sample):
@@ -308,6 +440,8 @@ def main():
         line["batched"] = bench_batched(args.batched_arrays, args.batched_steps)
         print("bench: batched workload scaled to 4 GiB of input", file=sys.stderr, flush=True)
         line["batched_4gib"] = bench_batched(args.batched_arrays, args.batched_steps, target_input_bytes=1 << 32)
+    if not args.no_serve:
+        line["serve"] = bench_serve(args.serve_seconds, args.serve_rate, args.serve_budget_gib)
     if not args.no_cpu:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(x_host, bits, block, min(args.cpu_sample, n))

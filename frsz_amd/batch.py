@@ -21,6 +21,7 @@ from .codec import compressed_size, dtype_code
 
 OP_COMPRESS = 0
 OP_DECOMPRESS = 1
+OP_GATHER = 2
 
 
 class Job(ctypes.Structure):
@@ -35,6 +36,8 @@ class Job(ctypes.Structure):
         ("out", ctypes.c_void_p),
         ("in_bytes", ctypes.c_uint64),
         ("out_bytes", ctypes.c_uint64),
+        ("idx", ctypes.c_void_p),
+        ("m", ctypes.c_uint64),
     ]
 
 
@@ -77,6 +80,12 @@ def decompress_job(stream: torch.Tensor, out: torch.Tensor, bits: int, block: in
     out_flat = out.view(-1)
     return (Job(OP_DECOMPRESS, dtype_code(out.dtype), bits, 0, block, out_flat.numel(), stream.data_ptr(),
                 out_flat.data_ptr(), stream.numel(), 0), (stream, out))
+
+
+def gather_job(stream: torch.Tensor, n: int, out: torch.Tensor, idx: torch.Tensor, bits: int, block: int):
+    idx = idx.view(-1)
+    return (Job(OP_GATHER, dtype_code(out.dtype), bits, 0, block, n, stream.data_ptr(), out.data_ptr(),
+                stream.numel(), 0, idx.data_ptr(), idx.numel()), (stream, out, idx))
 
 
 class BatchPlan:
@@ -128,4 +137,5 @@ class BatchPlan:
             pass
 
 
-__all__ = ["BatchPlan", "compress_job", "decompress_job", "compressed_size", "OP_COMPRESS", "OP_DECOMPRESS"]
+__all__ = ["BatchPlan", "compress_job", "decompress_job", "gather_job", "compressed_size", "OP_COMPRESS",
+           "OP_DECOMPRESS", "OP_GATHER"]

@@ -1,0 +1,63 @@
+// Internal interface of the persistent batch kernel, shared by the batch
+// plans (frsz2_batch.hip) and the serving runtime (frsz2_serve.hip).
+#pragma once
+
+#include "frsz2_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace frsz2 {
+
+// device descriptor of one job of the persistent kernel (96 bytes)
+struct bjob
+{
+  const void* in;           // values (compress) or stream (decompress, gather)
+  void* out;                // stream (compress) or values
+  const std::uint64_t* idx; // gather: indices
+  std::uint64_t n;          // values of the array / stream
+  std::uint64_t nb;
+  std::uint64_t R;
+  std::uint64_t tile0;      // first global tile of the job
+  std::uint64_t m;          // gather: indices
+  std::uint64_t B;
+  std::uint32_t CB;
+  int bits;
+  int lb;                   // log2 B, -1 if B is not a power of two (gather only)
+  int cls;
+  std::uint64_t pad_;
+};
+static_assert(sizeof(bjob) == 96, "descriptor layout");
+
+struct batch_shape
+{
+  int tile_shape;      // 0 narrow, 1 balanced, 2 wide
+  int workgroups;      // persistent grid
+  int tiles_per_claim; // 0 = static round-robin
+};
+
+// checks one job like the single-array entry points do
+int validate_job(const frsz2_job& j, int idx);
+
+// Appends the tile-path descriptors of jobs[0..n) to desc (grouped by kernel
+// class, job order within a class) with global tile numbers starting at 0,
+// and sets tiled[i] = 1 for the jobs they cover; returns the total tile
+// count.  The other jobs must run through run_plain.
+std::uint64_t build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vector<bjob>& desc,
+                                std::vector<char>& tiled);
+
+// Counter reset (when claiming) + persistent kernel over ndesc descriptors in
+// device memory, on stream s.
+hipError_t launch_persistent(const bjob* d_desc, int ndesc, std::uint64_t total_tiles,
+                             unsigned long long* d_counter, const batch_shape& shape, hipStream_t s);
+
+// One job as a plain single-array launch on s.
+int run_plain(const frsz2_job& j, hipStream_t s);
+
+// compute units of the current device (256 if the query fails)
+int device_cus();
+
+} // namespace frsz2

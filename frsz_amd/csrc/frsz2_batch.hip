@@ -23,6 +23,7 @@
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
 #include "tile.hpp"
+#include "batch_internal.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -39,38 +40,51 @@ using tile::WG;
 
 namespace {
 
-// device descriptor of one job (64 bytes)
-struct bjob
-{
-  const void* in;
-  void* out;
-  std::uint64_t n;
-  std::uint64_t nb;
-  std::uint64_t R;
-  std::uint64_t tile0;
-  std::uint32_t CB;
-  int bits;
-  int lb;
-  int cls;
-};
-static_assert(sizeof(bjob) == 64, "descriptor layout");
-
-// class = op * 14 + dtype_slot, dtype_slot in 0..13:
+// class = op * 14 + dtype_slot for compress / decompress, dtype_slot 0..13:
 //   float32: U8, U16, U32 x {aligned, packed} -> 0..5
 //   float64: U8, U16, U32, U64 x {aligned, packed} -> 6..13
+// gather: 28 + (float32 U8, U16, U32 -> 0..2; float64 U8..U64 -> 3..6)
 __host__ __device__ constexpr int
 class_of(int op, int dtype, int bits)
 {
   const int U = storage_bits(bits);
   const int ui = U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3;
+  if (op == FRSZ2_OP_GATHER) {
+    return 28 + (dtype == FRSZ2_FLOAT32 ? ui : 3 + ui);
+  }
   const int packed = bits < U ? 1 : 0;
   const int slot = dtype == FRSZ2_FLOAT32 ? ui * 2 + packed : 6 + ui * 2 + packed;
   return op * 14 + slot;
 }
 
+// indices per gather tile (256 threads x 4)
+constexpr int GATHER_TILE = WG * 4;
+
+// random-access decode of indices [t*GATHER_TILE, +GATHER_TILE) of a gather job
+template<typename T, int U>
+__device__ __forceinline__ void
+gather_tile(const bjob& J, std::uint64_t t)
+{
+  const layout L{ J.n, J.B, J.nb, J.R, J.CB, J.bits, U };
+  const auto* stream = static_cast<const std::uint8_t*>(J.in);
+  T* out = static_cast<T*>(J.out);
+  const std::uint64_t i0 = t * GATHER_TILE + static_cast<std::uint64_t>(threadIdx.x) * 4;
+  std::uint64_t idx[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    idx[v] = i0 + v < J.m ? J.idx[i0 + v] : ~std::uint64_t{ 0 };
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    if (i0 + v < J.m) {
+      out[i0 + v] = idx[v] < J.n ? decode_element<T, U>(stream, L, idx[v]) : T(0);
+    }
+  }
+}
+
 // Per-class tile shape, three settings:
 //   0 narrow:   4 values per thread for every class (smallest tiles, finest
-//               load balance, fewest registers)
+//               load balance, fewest registers), float64 decompress staged
 //   1 balanced: up to 8 values per thread, float64 decompress through the
 //               LDS output stage
 //   2 wide:     the shapes measured fastest for the plain kernels (up to 16
@@ -85,7 +99,7 @@ class_vpt(int cls, int shape)
   const bool f32 = slot < 6;
   const int ui = f32 ? slot / 2 : (slot - 6) / 2;
   const bool packed = (slot % 2) == 1;
-  if (shape == 0) {
+  if (shape == 0 || op == 2) {
     return 4;
   }
   if (op == 0) {
@@ -106,10 +120,10 @@ class_stage(int cls, int shape)
   const int slot = cls % 14;
   const bool f32 = slot < 6;
   const bool packed = (slot % 2) == 1;
-  if (shape == 0 || cls / 14 != 1 || packed) {
+  if (cls / 14 != 1 || cls >= 28 || packed) {
     return false;
   }
-  if (shape == 1) {
+  if (shape <= 1) {
     return !f32;
   }
   return !(f32 && slot / 2 == 2);
@@ -123,6 +137,7 @@ batch_lds_dwords()
 {
   return SHAPE == 0 ? tile::lds_dwords<4>() : tile::lds_dwords<8>();
 }
+static_assert(tile::stage_dwords<double, 4>() <= tile::lds_dwords<4>(), "stage fits");
 static_assert(tile::stage_dwords<float, 16>() <= tile::lds_dwords<8>(), "stage fits");
 static_assert(tile::stage_dwords<double, 8>() <= tile::lds_dwords<8>(), "stage fits");
 
@@ -130,7 +145,7 @@ template<int SHAPE>
 __device__ __forceinline__ void
 run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred)
 {
-  const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb, 0 };
+  const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb < 0 ? 0 : J.lb, 0 };
   auto* o8 = static_cast<std::uint8_t*>(J.out);
   const auto* i8 = static_cast<const std::uint8_t*>(J.in);
 #define FRSZ2_C(K, T, U, PK)                                                                                 \
@@ -174,6 +189,13 @@ run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred
     FRSZ2_D(25, double, 32, true);
     FRSZ2_D(26, double, 64, false);
     FRSZ2_D(27, double, 64, true);
+    case 28: gather_tile<float, 8>(J, t); break;
+    case 29: gather_tile<float, 16>(J, t); break;
+    case 30: gather_tile<float, 32>(J, t); break;
+    case 31: gather_tile<double, 8>(J, t); break;
+    case 32: gather_tile<double, 16>(J, t); break;
+    case 33: gather_tile<double, 32>(J, t); break;
+    case 34: gather_tile<double, 64>(J, t); break;
     default:
       break;
   }
@@ -272,11 +294,157 @@ aligned16(const void* p)
 
 } // namespace frsz2
 
+namespace frsz2 {
+
+int
+validate_job(const frsz2_job& j, int idx)
+{
+  const std::string tag = "job " + std::to_string(idx) + ": ";
+  if (j.op != FRSZ2_OP_COMPRESS && j.op != FRSZ2_OP_DECOMPRESS && j.op != FRSZ2_OP_GATHER) {
+    return set_error(FRSZ2_EARG, tag + "op must be compress, decompress or gather");
+  }
+  int rc = check_params(j.dtype, j.bits, j.block);
+  if (rc) {
+    return rc;
+  }
+  const layout L = layout::make(j.bits, j.block, j.n);
+  if (j.op == FRSZ2_OP_COMPRESS && j.out_bytes < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, tag + "output capacity below stream size");
+  }
+  if (j.op != FRSZ2_OP_COMPRESS && j.in_bytes < L.total_bytes()) {
+    return set_error(FRSZ2_ECAPACITY, tag + "input stream below stream size");
+  }
+  const std::uintptr_t vs = j.dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  const void* vals = j.op == FRSZ2_OP_COMPRESS ? j.in : j.out;
+  const void* strm = j.op == FRSZ2_OP_COMPRESS ? j.out : j.in;
+  const std::uint64_t nvals = j.op == FRSZ2_OP_GATHER ? j.m : j.n;
+  if (!strm || (nvals > 0 && !vals) || (reinterpret_cast<std::uintptr_t>(vals) & (vs - 1)) ||
+      (reinterpret_cast<std::uintptr_t>(strm) & 7u) ||
+      (j.op == FRSZ2_OP_GATHER && j.m > 0 && (!j.idx || (reinterpret_cast<std::uintptr_t>(j.idx) & 7u)))) {
+    return set_error(FRSZ2_EARG, tag + "null or misaligned pointer");
+  }
+  return FRSZ2_OK;
+}
+
+std::uint64_t
+build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vector<bjob>& desc,
+                  std::vector<char>& tiled)
+{
+  tiled.assign(n, 0);
+  std::uint64_t tiles = 0;
+  // jobs of one class get consecutive tiles, so a workgroup's successive
+  // claims mostly run the same code path of the kernel
+  std::vector<std::size_t> order(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
+    return class_of(jobs[a].op, jobs[a].dtype, jobs[a].bits) < class_of(jobs[b].op, jobs[b].dtype, jobs[b].bits);
+  });
+  for (const std::size_t i : order) {
+    const frsz2_job& j = jobs[i];
+    const int cls = class_of(j.op, j.dtype, j.bits);
+    const layout L = layout::make(j.bits, j.block, j.n);
+    const int lb = ilog2_exact(j.block);
+    std::uint64_t nt;
+    if (j.op == FRSZ2_OP_GATHER) {
+      nt = (j.m + GATHER_TILE - 1) / GATHER_TILE;
+    } else {
+      const std::uint64_t TILE = static_cast<std::uint64_t>(WG) * class_vpt(cls, tile_shape);
+      if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
+        continue;
+      }
+      nt = (j.n + TILE - 1) / TILE;
+      if (j.op == FRSZ2_OP_COMPRESS && nt == 0) {
+        nt = 1; // the pad of an empty stream is still written
+      }
+    }
+    tiled[i] = 1;
+    if (nt == 0) {
+      continue;
+    }
+    bjob d{};
+    d.in = j.in;
+    d.out = j.out;
+    d.idx = j.idx;
+    d.n = j.n;
+    d.nb = L.nb;
+    d.R = L.R;
+    d.tile0 = tiles;
+    d.m = j.m;
+    d.B = j.block;
+    d.CB = static_cast<std::uint32_t>(L.CB);
+    d.bits = j.bits;
+    d.lb = lb;
+    d.cls = cls;
+    desc.push_back(d);
+    tiles += nt;
+  }
+  return tiles;
+}
+
+hipError_t
+launch_persistent(const bjob* d_desc, int ndesc, std::uint64_t total_tiles, unsigned long long* d_counter,
+                  const batch_shape& shape, hipStream_t s)
+{
+  if (ndesc == 0 || total_tiles == 0) {
+    return hipSuccess;
+  }
+  const int chunk = shape.tiles_per_claim;
+  std::uint64_t grid = static_cast<std::uint64_t>(shape.workgroups);
+  if (grid > total_tiles) {
+    grid = total_tiles;
+  }
+  if (chunk > 0) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
+  }
+  const dim3 gd(static_cast<unsigned>(grid));
+  switch (shape.tile_shape) {
+    case 0:
+      hipLaunchKernelGGL((k_batch<0>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_batch<1>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);
+      break;
+    default:
+      hipLaunchKernelGGL((k_batch<2>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);
+      break;
+  }
+  return hipGetLastError();
+}
+
+int
+run_plain(const frsz2_job& j, hipStream_t s)
+{
+  switch (j.op) {
+    case FRSZ2_OP_COMPRESS:
+      return compress_device(j.dtype, j.bits, j.block, j.in, j.n, j.out, j.out_bytes, s);
+    case FRSZ2_OP_DECOMPRESS:
+      return decompress_device(j.dtype, j.bits, j.block, j.in, j.in_bytes, j.n, j.out, s);
+    default:
+      return gather_device(j.dtype, j.bits, j.block, j.in, j.in_bytes, j.n, j.idx, j.m, j.out, s);
+  }
+}
+
+int
+device_cus()
+{
+  int dev = 0;
+  int cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) {
+    return cus;
+  }
+  return 256;
+}
+
+} // namespace frsz2
+
 struct frsz2_batch_plan
 {
-  std::vector<frsz2_job> jobs;         // as given
-  std::vector<int> tiled;              // job index -> 1 if it takes the tile path
-  std::vector<frsz2::bjob> host_desc;  // per values-per-thread setting, rebuilt on change
+  std::vector<frsz2_job> jobs;        // as given
+  std::vector<char> tiled;            // job index -> 1 if it takes the persistent kernel
+  std::vector<frsz2::bjob> host_desc; // descriptors for cfg.tile_shape
   frsz2::bjob* d_desc = nullptr;
   unsigned long long* d_counter = nullptr;
   int ndesc = 0;
@@ -290,72 +458,12 @@ struct frsz2_batch_plan
 namespace frsz2 {
 namespace {
 
+// (re)build and upload the descriptors for cfg.tile_shape
 int
-validate_job(const frsz2_job& j, int idx)
-{
-  if (j.op != FRSZ2_OP_COMPRESS && j.op != FRSZ2_OP_DECOMPRESS) {
-    return set_error(FRSZ2_EARG, "job " + std::to_string(idx) + ": op must be compress or decompress");
-  }
-  int rc = check_params(j.dtype, j.bits, j.block);
-  if (rc) {
-    return rc;
-  }
-  const layout L = layout::make(j.bits, j.block, j.n);
-  if (j.op == FRSZ2_OP_COMPRESS && j.out_bytes < L.total_bytes()) {
-    return set_error(FRSZ2_ECAPACITY, "job " + std::to_string(idx) + ": output capacity below stream size");
-  }
-  if (j.op == FRSZ2_OP_DECOMPRESS && j.in_bytes < L.total_bytes()) {
-    return set_error(FRSZ2_ECAPACITY, "job " + std::to_string(idx) + ": input stream below stream size");
-  }
-  const std::uintptr_t vs = j.dtype == FRSZ2_FLOAT32 ? 4 : 8;
-  const void* vals = j.op == FRSZ2_OP_COMPRESS ? j.in : j.out;
-  const void* strm = j.op == FRSZ2_OP_COMPRESS ? j.out : j.in;
-  if (!strm || (j.n > 0 && !vals) || (reinterpret_cast<std::uintptr_t>(vals) & (vs - 1)) ||
-      (reinterpret_cast<std::uintptr_t>(strm) & 7u)) {
-    return set_error(FRSZ2_EARG, "job " + std::to_string(idx) + ": null or misaligned pointer");
-  }
-  return FRSZ2_OK;
-}
-
-// (re)build the device descriptors for cfg.tile_shape
-int
-build_descriptors(frsz2_batch_plan* P)
+upload_descriptors(frsz2_batch_plan* P)
 {
   P->host_desc.clear();
-  P->tiled.assign(P->jobs.size(), 0);
-  std::uint64_t tiles = 0;
-  for (std::size_t i = 0; i < P->jobs.size(); ++i) {
-    const frsz2_job& j = P->jobs[i];
-    const int cls = class_of(j.op, j.dtype, j.bits);
-    const std::uint64_t TILE = static_cast<std::uint64_t>(WG) * class_vpt(cls, P->cfg.tile_shape);
-    const int lb = ilog2_exact(j.block);
-    if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
-      continue;
-    }
-    const layout L = layout::make(j.bits, j.block, j.n);
-    std::uint64_t nt = (j.n + TILE - 1) / TILE;
-    if (j.op == FRSZ2_OP_COMPRESS && nt == 0) {
-      nt = 1; // the pad of an empty stream is still written
-    }
-    if (nt == 0) {
-      continue;
-    }
-    P->tiled[i] = 1;
-    bjob d{};
-    d.in = j.in;
-    d.out = j.out;
-    d.n = j.n;
-    d.nb = L.nb;
-    d.R = L.R;
-    d.tile0 = tiles;
-    d.CB = static_cast<std::uint32_t>(L.CB);
-    d.bits = j.bits;
-    d.lb = lb;
-    d.cls = cls;
-    P->host_desc.push_back(d);
-    tiles += nt;
-  }
-  P->total_tiles = tiles;
+  P->total_tiles = build_descriptors(P->jobs.data(), P->jobs.size(), P->cfg.tile_shape, P->host_desc, P->tiled);
   if (P->d_desc) {
     (void)hipFree(P->d_desc);
     P->d_desc = nullptr;
@@ -380,58 +488,25 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
 {
   if (P->cfg.plain_launches) {
     for (const frsz2_job& j : P->jobs) {
-      const int rc = j.op == FRSZ2_OP_COMPRESS
-                       ? compress_device(j.dtype, j.bits, j.block, j.in, j.n, j.out, j.out_bytes, s)
-                       : decompress_device(j.dtype, j.bits, j.block, j.in, j.in_bytes, j.n, j.out, s);
+      const int rc = run_plain(j, s);
       if (rc) {
         return rc;
       }
     }
     return FRSZ2_OK;
   }
-  if (P->ndesc > 0) {
-    const int chunk = P->cfg.tiles_per_claim;
-    std::uint64_t grid = static_cast<std::uint64_t>(P->cfg.workgroups);
-    if (chunk == 0 && grid > P->total_tiles) {
-      grid = P->total_tiles;
-    }
-    if (chunk > 0) {
-      hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, P->d_counter);
-    }
-    const dim3 gd(static_cast<unsigned>(grid));
-    switch (P->cfg.tile_shape) {
-      case 0:
-        hipLaunchKernelGGL((k_batch<0>), gd, dim3(WG), 0, s, P->d_desc, P->ndesc, P->total_tiles, P->d_counter,
-                           chunk);
-        break;
-      case 1:
-        hipLaunchKernelGGL((k_batch<1>), gd, dim3(WG), 0, s, P->d_desc, P->ndesc, P->total_tiles, P->d_counter,
-                           chunk);
-        break;
-      default:
-        hipLaunchKernelGGL((k_batch<2>), gd, dim3(WG), 0, s, P->d_desc, P->ndesc, P->total_tiles, P->d_counter,
-                           chunk);
-        break;
-    }
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-      return hip_status(e, "batch launch");
-    }
+  const batch_shape shape{ P->cfg.tile_shape, P->cfg.workgroups, P->cfg.tiles_per_claim };
+  const hipError_t e = launch_persistent(P->d_desc, P->ndesc, P->total_tiles, P->d_counter, shape, s);
+  if (e != hipSuccess) {
+    return hip_status(e, "batch launch");
   }
   // jobs outside the tile path: plain launches on the same stream
   for (std::size_t i = 0; i < P->jobs.size(); ++i) {
-    if (P->tiled[i]) {
-      continue;
-    }
-    const frsz2_job& j = P->jobs[i];
-    int rc;
-    if (j.op == FRSZ2_OP_COMPRESS) {
-      rc = compress_device(j.dtype, j.bits, j.block, j.in, j.n, j.out, j.out_bytes, s);
-    } else {
-      rc = decompress_device(j.dtype, j.bits, j.block, j.in, j.in_bytes, j.n, j.out, s);
-    }
-    if (rc) {
-      return rc;
+    if (!P->tiled[i]) {
+      const int rc = run_plain(P->jobs[i], s);
+      if (rc) {
+        return rc;
+      }
     }
   }
   return FRSZ2_OK;
@@ -498,7 +573,7 @@ apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
   const bool rebuild = n.tile_shape != P->cfg.tile_shape || P->host_desc.empty();
   P->cfg = n;
   if (rebuild) {
-    const int rc = build_descriptors(P);
+    const int rc = upload_descriptors(P);
     if (rc) {
       return rc;
     }
@@ -541,13 +616,7 @@ frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config
   }
   auto* P = new frsz2_batch_plan();
   P->jobs.assign(jobs, jobs + njobs);
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) {
-      P->cus = cus;
-    }
-  }
+  P->cus = device_cus();
   hipError_t e = hipMalloc(&P->d_counter, sizeof(unsigned long long));
   if (e != hipSuccess) {
     destroy(P);

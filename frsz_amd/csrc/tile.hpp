@@ -158,7 +158,7 @@ unpack_words(const std::uint32_t (&w)[VPT * U / 32 > 0 ? VPT * U / 32 : 1], UW (
     }
   } else {
     constexpr int PER = 32 / U;
-    constexpr std::uint32_t M = U == 32 ? 0xffffffffu : ((1u << U) - 1u);
+    constexpr std::uint32_t M = U == 32 ? 0xffffffffu : ((1u << (U % 32)) - 1u);
 #pragma unroll
     for (int d = 0; d < VPT / PER; ++d) {
 #pragma unroll

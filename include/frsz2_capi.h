@@ -135,26 +135,30 @@ int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* fla
 /* ---- batched: many arrays in one persistent, graph-captured launch ----
  * No counterpart in the reference: its kernels take one array per launch
  * (include/frsz2.hpp:1258-1288).  Each job has the meaning of one
- * frsz2_compress / frsz2_decompress call (same checks, same bytes). */
+ * frsz2_compress / frsz2_decompress / frsz2_gather call (same checks, same
+ * bytes). */
 
 enum
 {
   FRSZ2_OP_COMPRESS = 0,
-  FRSZ2_OP_DECOMPRESS = 1
+  FRSZ2_OP_DECOMPRESS = 1,
+  FRSZ2_OP_GATHER = 2
 };
 
 typedef struct frsz2_job
 {
-  int op;             /* FRSZ2_OP_COMPRESS or FRSZ2_OP_DECOMPRESS */
-  int dtype;          /* FRSZ2_FLOAT32 / FRSZ2_FLOAT64 */
+  int op;               /* FRSZ2_OP_COMPRESS, _DECOMPRESS or _GATHER */
+  int dtype;            /* FRSZ2_FLOAT32 / FRSZ2_FLOAT64 */
   int bits;
   int reserved;
-  uint64_t block;     /* exponent block size B */
-  uint64_t n;         /* values */
-  const void* in;     /* compress: values; decompress: stream (device memory) */
-  void* out;          /* compress: stream; decompress: values (device memory) */
-  uint64_t in_bytes;  /* decompress: bytes available at in */
-  uint64_t out_bytes; /* compress: capacity at out */
+  uint64_t block;       /* exponent block size B */
+  uint64_t n;           /* values of the array / stream */
+  const void* in;       /* compress: values; decompress, gather: stream (device memory) */
+  void* out;            /* compress: stream; decompress: n values; gather: m values */
+  uint64_t in_bytes;    /* decompress, gather: bytes available at in */
+  uint64_t out_bytes;   /* compress: capacity at out */
+  const uint64_t* idx;  /* gather: m indices (device memory) */
+  uint64_t m;           /* gather: index count */
 } frsz2_job;
 
 typedef struct frsz2_batch_config
@@ -188,6 +192,89 @@ int frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps
 int frsz2_batch_get_config(const frsz2_batch_plan* plan, frsz2_batch_config* cfg);
 int frsz2_batch_set_config(frsz2_batch_plan* plan, const frsz2_batch_config* cfg);
 int frsz2_batch_destroy(frsz2_batch_plan* plan);
+
+/* ---- serving: resident streams, HBM-budget admission, continuous batching ----
+ * No counterpart in the reference (its API is one synchronous call per
+ * array).  A server owns a device pool of hbm_budget bytes from which every
+ * request's output is allocated; a scheduler thread admits requests in
+ * arrival order while their outputs fit, and runs each group of admitted
+ * requests as one launch of the persistent batch kernel on one of `slots`
+ * streams, so requests arriving while others run are admitted into the next
+ * launch.  A request computes exactly what frsz2_compress / frsz2_decompress
+ * / frsz2_gather compute. */
+
+typedef struct frsz2_server frsz2_server;
+
+typedef struct frsz2_server_config
+{
+  uint64_t hbm_budget; /* bytes of the device pool (0 = 64 GiB) */
+  int slots;           /* concurrent launches (0 = 4) */
+  int max_batch;       /* requests per launch (0 = 1024) */
+  int tile_shape;      /* batch kernel tile shape (see frsz2_batch_config) */
+  int workgroups;      /* persistent grid (0 = 4 per CU) */
+  int tiles_per_claim; /* 0 = static round-robin */
+  int reserved;
+} frsz2_server_config;
+
+enum
+{
+  FRSZ2_REQ_DISCARD = 1 /* free the output as soon as it is no longer needed */
+};
+
+typedef struct frsz2_request
+{
+  int op;              /* FRSZ2_OP_COMPRESS, _DECOMPRESS or _GATHER */
+  int dtype;
+  int bits;
+  int flags;           /* FRSZ2_REQ_DISCARD */
+  uint64_t block;
+  uint64_t n;          /* values of the array / of the stream read */
+  const void* values;  /* compress: n input values (device memory) */
+  uint64_t source;     /* decompress, gather: ticket of the compress request whose
+                          stream is read (replay: index of that request in the trace) */
+  const uint64_t* idx; /* gather: m indices (device memory) */
+  uint64_t m;
+  double arrival;      /* replay: release time in seconds after the replay starts */
+} frsz2_request;
+
+typedef struct frsz2_result
+{
+  int status;
+  int reserved;
+  double t_arrival, t_admit, t_launch, t_done; /* seconds (server epoch; replay: replay start) */
+  void* out;           /* the output (stream or values) in the pool, while held */
+  uint64_t out_bytes;
+} frsz2_result;
+
+typedef struct frsz2_server_stats
+{
+  uint64_t launches;
+  uint64_t launched_requests;
+  uint64_t completed;
+  uint64_t pool_used;
+  uint64_t pool_peak;
+  uint64_t max_queue;
+} frsz2_server_stats;
+
+int frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** server);
+/* Queue a request (thread-safe).  Its output stays in the pool until
+ * frsz2_server_release, unless FRSZ2_REQ_DISCARD; a compress stream also
+ * stays while queued requests read it. */
+int frsz2_server_submit(frsz2_server* server, const frsz2_request* req, uint64_t* ticket);
+/* Block until the request completes; fills *res. */
+int frsz2_server_wait(frsz2_server* server, uint64_t ticket, frsz2_result* res);
+/* Drop the client's hold on a request's output. */
+int frsz2_server_release(frsz2_server* server, uint64_t ticket);
+/* Open-loop replay of a trace: request i is queued reqs[i].arrival seconds
+ * after the call; readers name their compress request by trace index; each
+ * stream stays resident until the last request of the trace reading it has
+ * completed; blocks until every request has completed.  tickets (optional)
+ * receives the ticket of every request whose output is still held (0 for
+ * FRSZ2_REQ_DISCARD). */
+int frsz2_server_replay(frsz2_server* server, const frsz2_request* reqs, size_t count, frsz2_result* results,
+                        uint64_t* tickets);
+int frsz2_server_get_stats(frsz2_server* server, frsz2_server_stats* stats);
+int frsz2_server_destroy(frsz2_server* server);
 
 /* Host-memory variants: stage through device memory, run the HIP kernels and
  * copy the result back; synchronous.  They keep the host-in / host-out

@@ -1,0 +1,134 @@
+"""Serving runtime: every request served in a mix (continuous batching,
+HBM-budget admission, requests overlapping on the device) returns exactly
+the bytes the same request returns alone through the single-array API."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def mods():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frsz_amd
+    from frsz_amd import serve
+
+    return frsz_amd, serve
+
+
+def arrays(seed, count):
+    from frsz_amd.workloads import smooth_signal_torch
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        f64 = bool(rng.random() < 0.5)
+        dt = torch.float64 if f64 else torch.float32
+        bits = int(rng.choice([8, 12, 16, 21, 32] + ([40, 64] if f64 else [])))
+        B = int(rng.choice([16, 32, 64, 5, 1]))
+        n = int(np.exp(rng.uniform(np.log(10), np.log(400_000))))
+        x = smooth_signal_torch(n, seed=500 + i, dtype=dt) * float(2.0 ** rng.integers(-10, 10))
+        out.append((x, bits, B))
+    return out
+
+
+def test_mix_equals_alone(mods):
+    fz, serve = mods
+    specs = arrays(1, 40)
+    rng = np.random.default_rng(2)
+    with serve.Server(hbm_budget=1 << 30, slots=3) as srv:
+        comp, dec, gat = [], [], []
+        for x, bits, B in specs:
+            t = srv.submit_compress(x, bits, B)
+            comp.append(t)
+            # readers submitted right behind their stream: they wait for it
+            dec.append(srv.submit_decompress(t, x.numel(), x.dtype, bits, B))
+            idx = torch.from_numpy(rng.integers(0, x.numel(), int(rng.integers(1, 4097)))).cuda()
+            gat.append((srv.submit_gather(t, x.numel(), x.dtype, bits, B, idx), idx))
+        for (x, bits, B), tc, td, (tg, idx) in zip(specs, comp, dec, gat):
+            alone_s = fz.compress(x, bits, B)
+            alone_y = fz.decompress(alone_s, x.numel(), x.dtype, bits, B)
+            alone_g = fz.gather(alone_s, x.numel(), x.dtype, bits, B, idx)
+            rc = srv.wait(tc)
+            assert rc["status"] == 0
+            assert torch.equal(srv.output(rc, torch.uint8, alone_s.numel()), alone_s)
+            rd = srv.wait(td)
+            assert torch.equal(srv.output(rd, x.dtype, x.numel()).view(torch.uint8), alone_y.view(torch.uint8))
+            rg = srv.wait(tg)
+            assert torch.equal(srv.output(rg, x.dtype, idx.numel()).view(torch.uint8), alone_g.view(torch.uint8))
+            assert rc["t_arrival"] <= rc["t_admit"] <= rc["t_launch"] <= rc["t_done"]
+            for t in (tg, td, tc):
+                srv.release(t)
+        st = srv.stats()
+        assert st["completed"] == 120 and st["pool_used"] == 0
+        assert st["launched_requests"] == 120
+
+
+def test_budget_admission_and_release(mods):
+    """Outputs that together exceed the budget are admitted as memory frees."""
+    fz, serve = mods
+    x = torch.randn(200_000, device="cuda", dtype=torch.float32)
+    size = fz.compressed_size(torch.float32, 32, 32, x.numel())
+    slot = -(-size // 256) * 256  # pool allocations are 256-byte granular
+    with serve.Server(hbm_budget=3 * slot + 256, slots=2) as srv:
+        big = torch.randn(4 * size, device="cuda", dtype=torch.float32)
+        with pytest.raises(fz.Frsz2Error) as e:
+            srv.submit_compress(big, 32, 32)
+        assert e.value.code == 3
+        held = [srv.submit_compress(x, 32, 32) for _ in range(3)]
+        late = srv.submit_compress(x * 2, 32, 32)  # no room until one is released
+        for t in held:
+            srv.wait(t)
+        assert srv.stats()["pool_used"] >= 3 * size - 3 * 256
+        srv.release(held[0])
+        r = srv.wait(late)
+        assert torch.equal(srv.output(r, torch.uint8, size), fz.compress(x * 2, 32, 32))
+        for t in held[1:] + [late]:
+            srv.release(t)
+
+
+def test_replay_trace_equals_alone(mods):
+    fz, serve = mods
+    specs = arrays(3, 24)
+    rng = np.random.default_rng(4)
+    trace = np.zeros(0, dtype=serve.REQUEST_DTYPE)
+    rows, keep_alive, meta = [], [], []
+    t = 0.0
+    for i, (x, bits, B) in enumerate(specs):
+        t += float(rng.exponential(2e-4))
+        ci = len(rows)
+        rows.append((0, int(x.dtype == torch.float64), bits, 0, B, x.numel(), x.data_ptr(), 0, 0, 0, t))
+        meta.append(("c", i, None))
+        keep_alive.append(x)
+        t += float(rng.exponential(2e-4))
+        rows.append((1, int(x.dtype == torch.float64), bits, 0, B, x.numel(), 0, ci, 0, 0, t))
+        meta.append(("d", i, None))
+        idx = torch.from_numpy(rng.integers(0, x.numel(), 100)).cuda()
+        keep_alive.append(idx)
+        t += float(rng.exponential(2e-4))
+        rows.append((2, int(x.dtype == torch.float64), bits, 0, B, x.numel(), 0, ci, idx.data_ptr(), 100, t))
+        meta.append(("g", i, idx))
+    trace = np.array(rows, dtype=serve.REQUEST_DTYPE)
+    with serve.Server(hbm_budget=1 << 30, slots=4) as srv:
+        res, tickets = srv.replay(trace)
+        assert np.all(res["status"] == 0)
+        assert np.all(res["t_done"] >= res["t_arrival"])
+        for r, tk, (kind, i, idx) in zip(res, tickets, meta):
+            x, bits, B = specs[i]
+            s = fz.compress(x, bits, B)
+            rd = {k: r[k] for k in r.dtype.names}
+            if kind == "c":
+                got, want = srv.output(rd, torch.uint8, s.numel()), s
+            elif kind == "d":
+                got = srv.output(rd, x.dtype, x.numel()).view(torch.uint8)
+                want = fz.decompress(s, x.numel(), x.dtype, bits, B).view(torch.uint8)
+            else:
+                got = srv.output(rd, x.dtype, idx.numel()).view(torch.uint8)
+                want = fz.gather(s, x.numel(), x.dtype, bits, B, idx).view(torch.uint8)
+            assert torch.equal(got, want), (kind, i)
+        for tk in tickets:
+            srv.release(int(tk))
+        assert srv.stats()["pool_used"] == 0
