@@ -194,6 +194,7 @@ struct frsz2_server
   std::uint64_t completed = 0;
   std::uint64_t max_queue = 0;
   int replay_pending = 0;
+  frsz2::clk::time_point last_progress;
 };
 
 namespace frsz2 {
@@ -438,6 +439,20 @@ scheduler(frsz2_server* S)
         progressed = true;
       }
     }
+    if (progressed) {
+      S->last_progress = clk::now();
+    } else if (!any_busy && !S->queue.empty() && S->cfg.stall_ms > 0 &&
+               clk::now() - S->last_progress > std::chrono::milliseconds(S->cfg.stall_ms)) {
+      // nothing runs and nothing can be admitted (the pool is held by
+      // outputs that are not being released): fail the oldest request
+      const std::uint64_t t = S->queue.front();
+      S->queue.pop_front();
+      set_error(FRSZ2_ECAPACITY, "request could not be admitted: HBM pool held by unreleased outputs");
+      finish(S, t, FRSZ2_ECAPACITY, seconds_since(S, clk::now()));
+      S->last_progress = clk::now();
+      S->cv.notify_all();
+      continue;
+    }
     if (S->stop && S->queue.empty() && !any_busy) {
       break;
     }
@@ -454,9 +469,13 @@ scheduler(frsz2_server* S)
   }
 }
 
+// extra_refs: holds on the output besides the client's (replay: the trace's
+// readers of a compress); count_reader: a decompress / gather takes a hold on
+// its stream (false in replay, where the stream's readers were counted up
+// front)
 int
-submit_locked(frsz2_server* S, const frsz2_request& r, std::uint64_t source, int extra_refs, frsz2_result* sink,
-              double arrival, std::uint64_t* ticket)
+submit_locked(frsz2_server* S, const frsz2_request& r, std::uint64_t source, int extra_refs, bool count_reader,
+              frsz2_result* sink, double arrival, std::uint64_t* ticket)
 {
   const std::uint64_t t = S->next_ticket++;
   record& q = S->recs[t];
@@ -492,12 +511,15 @@ submit_locked(frsz2_server* S, const frsz2_request& r, std::uint64_t source, int
         (s->second.state == DONE && !s->second.out_live) || s->second.r.dtype != r.dtype ||
         s->second.r.bits != r.bits || s->second.r.block != r.block || s->second.r.n != r.n) {
       rc = set_error(FRSZ2_EARG, "source is not a resident stream with the same dtype, bits, block and n");
-    } else {
+    } else if (count_reader) {
       ++s->second.refs;
     }
   }
   if (rc != FRSZ2_OK) {
     if (r.op != FRSZ2_OP_COMPRESS) {
+      if (!count_reader && source) {
+        unref(S, source); // the hold counted up front for this reader
+      }
       q.source = 0;
     }
     q.client_hold = false;
@@ -582,6 +604,9 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
   if (S->cfg.tiles_per_claim < 0) {
     S->cfg.tiles_per_claim = 0;
   }
+  if (S->cfg.stall_ms == 0) {
+    S->cfg.stall_ms = 10000;
+  }
   hipError_t e = hipMalloc(&S->pool, S->cfg.hbm_budget);
   if (e != hipSuccess) {
     S->pool = nullptr;
@@ -602,6 +627,7 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
     }
   }
   S->epoch = clk::now();
+  S->last_progress = S->epoch;
   S->worker = std::thread(scheduler, S);
   *out = S;
   return FRSZ2_OK;
@@ -618,7 +644,7 @@ frsz2_server_submit(frsz2_server* S, const frsz2_request* req, uint64_t* ticket)
   int rc;
   {
     std::lock_guard<std::mutex> g(S->mu);
-    rc = submit_locked(S, *req, req->source, 0, nullptr, seconds_since(S, clk::now()), ticket);
+    rc = submit_locked(S, *req, req->source, 0, true, nullptr, seconds_since(S, clk::now()), ticket);
   }
   S->cv.notify_all();
   return rc;
@@ -713,7 +739,7 @@ frsz2_server_replay(frsz2_server* S, const frsz2_request* reqs, size_t count, fr
     {
       std::lock_guard<std::mutex> g(S->mu);
       const uint64_t src = reqs[i].op == FRSZ2_OP_COMPRESS ? 0 : tk[reqs[i].source];
-      (void)submit_locked(S, reqs[i], src, readers[i], &results[i], base + reqs[i].arrival, &tk[i]);
+      (void)submit_locked(S, reqs[i], src, readers[i], false, &results[i], base + reqs[i].arrival, &tk[i]);
     }
     S->cv.notify_all();
   }

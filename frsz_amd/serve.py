@@ -30,7 +30,7 @@ REQ_DISCARD = 1
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("slots", ctypes.c_int), ("max_batch", ctypes.c_int),
                 ("tile_shape", ctypes.c_int), ("workgroups", ctypes.c_int), ("tiles_per_claim", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("stall_ms", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -136,7 +136,7 @@ class Server:
         if nbytes > result["out_bytes"]:
             raise ValueError("output is smaller than requested")
         torch.cuda.synchronize()
-        if _hip.hipMemcpy(out.data_ptr(), result["out"], nbytes, _HIP_D2D) != 0:
+        if _hip.hipMemcpy(out.data_ptr(), int(result["out"]), nbytes, _HIP_D2D) != 0:
             raise RuntimeError("hipMemcpy failed")
         return out
 

@@ -213,7 +213,9 @@ typedef struct frsz2_server_config
   int tile_shape;      /* batch kernel tile shape (see frsz2_batch_config) */
   int workgroups;      /* persistent grid (0 = 4 per CU) */
   int tiles_per_claim; /* 0 = static round-robin */
-  int reserved;
+  int stall_ms;        /* when nothing runs and the oldest queued request has
+                          not fit the pool for this long, it fails with
+                          FRSZ2_ECAPACITY (0 = 10000; negative = never) */
 } frsz2_server_config;
 
 enum
