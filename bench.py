@@ -312,6 +312,21 @@ def cpu_baseline(x_host, bits, block, sample):
     return res
 
 
+def profiled_traffic(op, side, bits, block):
+    """HBM bytes per launch of the plain kernel for `op`, from the committed
+    FETCH_SIZE / WRITE_SIZE counter summary of this exact workload (None for
+    any other configuration)."""
+    if (side, bits, block) != (512, 32, 32):
+        return None
+    path = os.path.join(ROOT, "profiles", "plain_f64_512cubed_b32_B32_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        data = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        return int(data["kernels"][op]["traffic_bytes"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_model():
     try:
         for ln in open("/proc/cpuinfo"):
@@ -414,7 +429,8 @@ def main():
         "decompress_us": round(sum(td) / len(td) * 1e6, 2),
         "bit_exact": verified,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dom, side, bits, block), "algorithmic_bytes": alg},
     }
     if not args.no_copy:
         # a plain 1 GiB device copy (torch), the calibration kernel for the
