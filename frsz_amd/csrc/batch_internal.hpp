@@ -37,6 +37,7 @@ struct batch_shape
   int tile_shape;      // 0 narrow, 1 balanced, 2 wide
   int workgroups;      // persistent grid
   int tiles_per_claim; // 0 = static round-robin
+  int ops;             // directions present (1 compress, 2 decompress, 4 gather); 0 = any
 };
 
 // checks one job like the single-array entry points do

@@ -141,7 +141,10 @@ static_assert(tile::stage_dwords<double, 4>() <= tile::lds_dwords<4>(), "stage f
 static_assert(tile::stage_dwords<float, 16>() <= tile::lds_dwords<8>(), "stage fits");
 static_assert(tile::stage_dwords<double, 8>() <= tile::lds_dwords<8>(), "stage fits");
 
-template<int SHAPE>
+// OPS: bit mask of the directions a kernel instance handles (1 compress,
+// 2 decompress, 4 gather); the code (and register allocation) of the other
+// classes is left out.
+template<int SHAPE, int OPS>
 __device__ __forceinline__ void
 run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred)
 {
@@ -150,13 +153,16 @@ run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred
   const auto* i8 = static_cast<const std::uint8_t*>(J.in);
 #define FRSZ2_C(K, T, U, PK)                                                                                 \
   case K:                                                                                                    \
+    if constexpr ((OPS & 1) != 0)                                                                            \
     tile::compress_tile<T, U, class_vpt(K, SHAPE), PK, true>(static_cast<const typename fmt<T>::uw*>(J.in),   \
                                                             o8, p, t, lds, wred);                            \
+    if constexpr ((OPS & 1) != 0)                                                                            \
     if (t == 0)                                                                                              \
       tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);                                   \
     break
 #define FRSZ2_D(K, T, U, PK)                                                                                 \
   case K:                                                                                                    \
+    if constexpr ((OPS & 2) != 0)                                                                            \
     tile::decompress_tile<T, U, class_vpt(K, SHAPE), PK, true, class_stage(K, SHAPE)>(                         \
       i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds, lds);                                         \
     break
@@ -189,13 +195,34 @@ run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred
     FRSZ2_D(25, double, 32, true);
     FRSZ2_D(26, double, 64, false);
     FRSZ2_D(27, double, 64, true);
-    case 28: gather_tile<float, 8>(J, t); break;
-    case 29: gather_tile<float, 16>(J, t); break;
-    case 30: gather_tile<float, 32>(J, t); break;
-    case 31: gather_tile<double, 8>(J, t); break;
-    case 32: gather_tile<double, 16>(J, t); break;
-    case 33: gather_tile<double, 32>(J, t); break;
-    case 34: gather_tile<double, 64>(J, t); break;
+    case 28:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<float, 8>(J, t);
+      break;
+    case 29:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<float, 16>(J, t);
+      break;
+    case 30:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<float, 32>(J, t);
+      break;
+    case 31:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<double, 8>(J, t);
+      break;
+    case 32:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<double, 16>(J, t);
+      break;
+    case 33:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<double, 32>(J, t);
+      break;
+    case 34:
+      if constexpr ((OPS & 4) != 0)
+        gather_tile<double, 64>(J, t);
+      break;
     default:
       break;
   }
@@ -223,7 +250,7 @@ find_job(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, int hint)
   return lo;
 }
 
-template<int SHAPE>
+template<int SHAPE, int OPS>
 __global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
                                               int njobs,
                                               std::uint64_t total_tiles,
@@ -260,7 +287,7 @@ __global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
     for (std::uint64_t g = g0; g < g1; ++g) {
       j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, j));
       const bjob J = jobs[j];
-      run_tile<SHAPE>(J, g - J.tile0, lds, wred);
+      run_tile<SHAPE, OPS>(J, g - J.tile0, lds, wred);
     }
   }
 }
@@ -399,17 +426,22 @@ launch_persistent(const bjob* d_desc, int ndesc, std::uint64_t total_tiles, unsi
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
   }
   const dim3 gd(static_cast<unsigned>(grid));
-  switch (shape.tile_shape) {
-    case 0:
-      hipLaunchKernelGGL((k_batch<0>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_batch<1>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);
-      break;
-    default:
-      hipLaunchKernelGGL((k_batch<2>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);
-      break;
+  // one kernel instance per (tile shape, set of directions present)
+  const int ops = shape.ops == 0 ? 7 : shape.ops == 1 ? 1 : (shape.ops & 1) != 0 ? 7 : 6;
+#define FRSZ2_KB(S, O)                                                                                       \
+  hipLaunchKernelGGL((k_batch<S, O>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk)
+  switch (shape.tile_shape * 8 + ops) {
+    case 0 * 8 + 1: FRSZ2_KB(0, 1); break;
+    case 0 * 8 + 6: FRSZ2_KB(0, 6); break;
+    case 0 * 8 + 7: FRSZ2_KB(0, 7); break;
+    case 1 * 8 + 1: FRSZ2_KB(1, 1); break;
+    case 1 * 8 + 6: FRSZ2_KB(1, 6); break;
+    case 1 * 8 + 7: FRSZ2_KB(1, 7); break;
+    case 2 * 8 + 1: FRSZ2_KB(2, 1); break;
+    case 2 * 8 + 6: FRSZ2_KB(2, 6); break;
+    default: FRSZ2_KB(2, 7); break;
   }
+#undef FRSZ2_KB
   return hipGetLastError();
 }
 
@@ -453,6 +485,7 @@ struct frsz2_batch_plan
   int cus = 256;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int ops = 0; // directions present among the jobs (1 compress, 2 decompress, 4 gather)
 };
 
 namespace frsz2 {
@@ -495,7 +528,7 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
     }
     return FRSZ2_OK;
   }
-  const batch_shape shape{ P->cfg.tile_shape, P->cfg.workgroups, P->cfg.tiles_per_claim };
+  const batch_shape shape{ P->cfg.tile_shape, P->cfg.workgroups, P->cfg.tiles_per_claim, P->ops };
   const hipError_t e = launch_persistent(P->d_desc, P->ndesc, P->total_tiles, P->d_counter, shape, s);
   if (e != hipSuccess) {
     return hip_status(e, "batch launch");
@@ -617,6 +650,9 @@ frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config
   auto* P = new frsz2_batch_plan();
   P->jobs.assign(jobs, jobs + njobs);
   P->cus = device_cus();
+  for (size_t i = 0; i < njobs; ++i) {
+    P->ops |= 1 << jobs[i].op;
+  }
   hipError_t e = hipMalloc(&P->d_counter, sizeof(unsigned long long));
   if (e != hipSuccess) {
     destroy(P);

@@ -366,7 +366,11 @@ launch_slot(frsz2_server* S, slot& sl)
     if (e != hipSuccess) {
       return hip_status(e, "hipMemcpyAsync");
     }
-    const batch_shape shape{ S->cfg.tile_shape, S->cfg.workgroups, S->cfg.tiles_per_claim };
+    int ops = 0;
+    for (const frsz2_job& j : sl.jobs) {
+      ops |= 1 << j.op;
+    }
+    const batch_shape shape{ S->cfg.tile_shape, S->cfg.workgroups, S->cfg.tiles_per_claim, ops };
     e = launch_persistent(sl.d_desc, nd, tiles, sl.d_counter, shape, sl.stream);
     if (e != hipSuccess) {
       return hip_status(e, "batch launch");
