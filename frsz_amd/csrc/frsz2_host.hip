@@ -22,7 +22,7 @@ namespace {
 thread_local std::string g_error;
 
 // tuning table of the tiled kernels: [op][dtype][U = 8,16,32,64][bits < U],
-// each entry vpt * 4 + stage * 2 + nt.  Defaults come from measurements on MI355X;
+// each entry vpt * 8 + linear * 4 + stage * 2 + nt.  Defaults come from measurements on MI355X;
 // frsz2_tune_set (the autotuner) overrides them per process.
 std::atomic<int> g_variant[2][2][4][2];
 std::once_flag g_variant_once;
@@ -36,9 +36,9 @@ u_index(int bits)
 
 // encoded table entry
 constexpr int
-venc(int vpt, int stage, int nt)
+venc(int vpt, int stage, int nt, int linear = 0)
 {
-  return vpt * 4 + stage * 2 + nt;
+  return vpt * 8 + linear * 4 + stage * 2 + nt;
 }
 
 // Defaults measured on MI355X with the plain kernels (1 GiB float64 field,
@@ -46,29 +46,30 @@ venc(int vpt, int stage, int nt)
 void
 fill_variants()
 {
+  // venc(values per thread, staged stores, non-temporal, linear tile order)
   static const int table[2][2][4][2] = {
     // compress
-    { // float32: U8, U16, U32 (U64 unused)
-      { { venc(8, 0, 0), venc(8, 0, 0) },
-        { venc(8, 0, 1), venc(8, 0, 0) },
-        { venc(4, 0, 1), venc(8, 0, 0) },
-        { venc(4, 0, 1), venc(4, 0, 1) } },
+    { // float32: U8, U16, U32 (U64 unused) x {aligned, packed}
+      { { venc(8, 0, 0, 1), venc(8, 0, 0, 0) },
+        { venc(8, 0, 1, 1), venc(8, 0, 0, 0) },
+        { venc(4, 0, 1, 1), venc(8, 0, 0, 0) },
+        { venc(4, 0, 1, 1), venc(4, 0, 1, 1) } },
       // float64
-      { { venc(4, 0, 1), venc(4, 0, 1) },
-        { venc(4, 0, 1), venc(4, 0, 1) },
-        { venc(4, 0, 1), venc(4, 0, 1) },
-        { venc(4, 0, 1), venc(4, 0, 1) } } },
+      { { venc(4, 0, 1, 1), venc(4, 0, 1, 0) },
+        { venc(4, 0, 1, 0), venc(4, 0, 1, 0) },
+        { venc(4, 0, 1, 1), venc(4, 0, 1, 0) },
+        { venc(4, 0, 1, 1), venc(4, 0, 1, 0) } } },
     // decompress
     { // float32
-      { { venc(16, 1, 1), venc(8, 0, 1) },
-        { venc(16, 1, 1), venc(8, 0, 1) },
-        { venc(8, 0, 1), venc(8, 0, 1) },
-        { venc(4, 0, 1), venc(4, 0, 1) } },
+      { { venc(16, 1, 1, 1), venc(8, 0, 1, 1) },
+        { venc(16, 1, 1, 1), venc(8, 0, 1, 1) },
+        { venc(8, 0, 1, 1), venc(8, 0, 1, 1) },
+        { venc(4, 0, 1, 1), venc(4, 0, 1, 1) } },
       // float64
-      { { venc(8, 1, 1), venc(4, 0, 1) },
-        { venc(8, 1, 1), venc(4, 0, 1) },
-        { venc(8, 1, 1), venc(4, 0, 1) },
-        { venc(8, 1, 1), venc(4, 0, 1) } } }
+      { { venc(8, 1, 1, 1), venc(4, 0, 1, 1) },
+        { venc(8, 1, 1, 1), venc(4, 0, 1, 1) },
+        { venc(8, 1, 1, 1), venc(4, 0, 1, 1) },
+        { venc(8, 1, 1, 1), venc(4, 0, 1, 1) } } }
   };
   for (int op = 0; op < 2; ++op)
     for (int d = 0; d < 2; ++d)
@@ -89,15 +90,15 @@ get_variant(int op, int dtype, int bits)
 {
   init_variants();
   const int v = g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].load(std::memory_order_relaxed);
-  return variant{ v / 4, v % 2, (v / 2) % 2 };
+  return variant{ v / 8, v % 2, (v / 2) % 2, (v / 4) % 2 };
 }
 
 int
-set_variant(int op, int dtype, int bits, int vpt, int nt, int stage)
+set_variant(int op, int dtype, int bits, int vpt, int nt, int stage, int linear)
 {
   init_variants();
   g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].store(
-    vpt * 4 + (stage ? 2 : 0) + (nt ? 1 : 0), std::memory_order_relaxed);
+    venc(vpt, stage ? 1 : 0, nt ? 1 : 0, linear ? 1 : 0), std::memory_order_relaxed);
   return 0;
 }
 
@@ -169,7 +170,7 @@ frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags)
     return set_error(FRSZ2_EARG, "op must be 0/1 and values_per_thread 4, 8 or 16");
   }
   return set_variant(op, dtype, bits, values_per_thread, flags & FRSZ2_TUNE_NONTEMPORAL,
-                     op == OP_DECOMPRESS ? (flags & FRSZ2_TUNE_STAGED_STORES) : 0);
+                     op == OP_DECOMPRESS ? (flags & FRSZ2_TUNE_STAGED_STORES) : 0, flags & FRSZ2_TUNE_LINEAR_ORDER);
 }
 
 int
@@ -186,7 +187,8 @@ frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags)
   }
   const variant v = get_variant(op, dtype, bits);
   *values_per_thread = v.vpt;
-  *flags = (v.nt ? FRSZ2_TUNE_NONTEMPORAL : 0) | (v.stage ? FRSZ2_TUNE_STAGED_STORES : 0);
+  *flags = (v.nt ? FRSZ2_TUNE_NONTEMPORAL : 0) | (v.stage ? FRSZ2_TUNE_STAGED_STORES : 0) |
+           (v.linear ? FRSZ2_TUNE_LINEAR_ORDER : 0);
   return FRSZ2_OK;
 }
 

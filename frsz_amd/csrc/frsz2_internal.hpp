@@ -27,9 +27,10 @@ struct variant
   int vpt;
   int nt;
   int stage;
+  int linear; // tiles in workgroup order instead of one contiguous range per XCD
 };
 variant get_variant(int op, int dtype, int bits);
-int set_variant(int op, int dtype, int bits, int vpt, int nt, int stage);
+int set_variant(int op, int dtype, int bits, int vpt, int nt, int stage, int linear);
 int hip_status(hipError_t e, const char* what);
 
 int compress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t n, void* d_out,

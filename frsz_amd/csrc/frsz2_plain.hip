@@ -33,11 +33,12 @@ template<typename T, int U, int VPT, bool PACKED, bool NT>
 __global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw* __restrict__ in,
                                                       std::uint8_t* __restrict__ out,
                                                       tplan p,
-                                                      std::uint32_t ntiles)
+                                                      std::uint32_t ntiles,
+                                                      int linear)
 {
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
-  const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
+  const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
   tile::compress_tile<T, U, VPT, PACKED, NT>(in, out, p, t, lds, wred);
   if (t == 0) {
     tile::zero_pad(out, p.nb, U, p.R, static_cast<int>(threadIdx.x), WG);
@@ -48,11 +49,12 @@ template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE>
 __global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __restrict__ in,
                                                         typename fmt<T>::uw* __restrict__ out,
                                                         tplan p,
-                                                        std::uint32_t ntiles)
+                                                        std::uint32_t ntiles,
+                                                        int linear)
 {
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
   __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT>() : 1];
-  const std::uint32_t t = tile::xcd_tile(blockIdx.x, ntiles);
+  const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
   tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE>(in, out, p, t, lds, stage);
 }
 
@@ -153,7 +155,7 @@ grid_for(std::uint64_t work)
 
 template<typename T, int U, int VPT, bool NT>
 hipError_t
-launch_compress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
+launch_compress_tiled(const void* in, void* out, const layout& L, int lb, int linear, hipStream_t s)
 {
   constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
   tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0 };
@@ -165,17 +167,17 @@ launch_compress_tiled(const void* in, void* out, const layout& L, int lb, hipStr
   auto* o = static_cast<std::uint8_t*>(out);
   if (L.bits == U) {
     hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
-                       static_cast<std::uint32_t>(nt));
+                       static_cast<std::uint32_t>(nt), linear);
   } else {
     hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
-                       static_cast<std::uint32_t>(nt));
+                       static_cast<std::uint32_t>(nt), linear);
   }
   return hipGetLastError();
 }
 
 template<typename T, int U, int VPT, bool NT, bool STAGE>
 hipError_t
-launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipStream_t s)
+launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, int linear, hipStream_t s)
 {
   constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
   tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0 };
@@ -187,10 +189,10 @@ launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, hipS
   auto* o = static_cast<typename fmt<T>::uw*>(out);
   if (L.bits == U) {
     hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false, NT, STAGE>), dim3(nt), dim3(WG), 0, s, x, o, p,
-                       static_cast<std::uint32_t>(nt));
+                       static_cast<std::uint32_t>(nt), linear);
   } else {
     hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true, NT, STAGE>), dim3(nt), dim3(WG), 0, s, x, o, p,
-                       static_cast<std::uint32_t>(nt));
+                       static_cast<std::uint32_t>(nt), linear);
   }
   return hipGetLastError();
 }
@@ -240,17 +242,17 @@ run_compress(const void* in, void* out, const layout& L, bool tiled, int lb, hip
   if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
     switch (v.vpt * 2 + v.nt) {
       case 8:
-        return launch_compress_tiled<T, U, 4, false>(in, out, L, lb, s);
+        return launch_compress_tiled<T, U, 4, false>(in, out, L, lb, v.linear, s);
       case 9:
-        return launch_compress_tiled<T, U, 4, true>(in, out, L, lb, s);
+        return launch_compress_tiled<T, U, 4, true>(in, out, L, lb, v.linear, s);
       case 16:
-        return launch_compress_tiled<T, U, 8, false>(in, out, L, lb, s);
+        return launch_compress_tiled<T, U, 8, false>(in, out, L, lb, v.linear, s);
       case 17:
-        return launch_compress_tiled<T, U, 8, true>(in, out, L, lb, s);
+        return launch_compress_tiled<T, U, 8, true>(in, out, L, lb, v.linear, s);
       case 32:
-        return launch_compress_tiled<T, U, 16, false>(in, out, L, lb, s);
+        return launch_compress_tiled<T, U, 16, false>(in, out, L, lb, v.linear, s);
       default:
-        return launch_compress_tiled<T, U, 16, true>(in, out, L, lb, s);
+        return launch_compress_tiled<T, U, 16, true>(in, out, L, lb, v.linear, s);
     }
   }
   return launch_compress_generic<T, U>(in, out, L, s);
@@ -262,7 +264,7 @@ run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, h
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
-#define FRSZ2_DL(V, N, S) return launch_decompress_tiled<T, U, V, N, S>(in, out, L, lb, s)
+#define FRSZ2_DL(V, N, S) return launch_decompress_tiled<T, U, V, N, S>(in, out, L, lb, v.linear, s)
     switch (v.vpt * 4 + v.stage * 2 + v.nt) {
       case 16: FRSZ2_DL(4, false, false);
       case 17: FRSZ2_DL(4, true, false);

@@ -121,13 +121,16 @@ int frsz2_gather(int dtype,
  * that many values, so it also bounds the largest B the tiled path takes);
  * flags: FRSZ2_TUNE_NONTEMPORAL puts the non-temporal hint on the streaming
  * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) writes each tile's output
- * through LDS with whole-cache-line stores.  Process-wide.  No counterpart
+ * through LDS with whole-cache-line stores, FRSZ2_TUNE_LINEAR_ORDER takes
+ * tiles in workgroup order (default: one contiguous range of tiles per
+ * XCD).  Process-wide.  No counterpart
  * in the reference (its kernels have a fixed launch shape,
  * test/test_frsz_gpu.cu:325-330). */
 enum
 {
   FRSZ2_TUNE_NONTEMPORAL = 1,
-  FRSZ2_TUNE_STAGED_STORES = 2
+  FRSZ2_TUNE_STAGED_STORES = 2,
+  FRSZ2_TUNE_LINEAR_ORDER = 4
 };
 int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags);
 int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags);
