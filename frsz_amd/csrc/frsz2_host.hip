@@ -166,8 +166,8 @@ frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags)
     return rc;
   }
   if ((op != OP_COMPRESS && op != OP_DECOMPRESS) ||
-      (values_per_thread != 4 && values_per_thread != 8 && values_per_thread != 16)) {
-    return set_error(FRSZ2_EARG, "op must be 0/1 and values_per_thread 4, 8 or 16");
+      (values_per_thread != 2 && values_per_thread != 4 && values_per_thread != 8 && values_per_thread != 16)) {
+    return set_error(FRSZ2_EARG, "op must be 0/1 and values_per_thread 2, 4, 8 or 16");
   }
   return set_variant(op, dtype, bits, values_per_thread, flags & FRSZ2_TUNE_NONTEMPORAL,
                      op == OP_DECOMPRESS ? (flags & FRSZ2_TUNE_STAGED_STORES) : 0, flags & FRSZ2_TUNE_LINEAR_ORDER);

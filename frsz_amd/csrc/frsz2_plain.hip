@@ -231,29 +231,51 @@ launch_gather(const void* in, const layout& L, const std::uint64_t* idx, std::ui
   return hipGetLastError();
 }
 
-// Tiled launch with the (values per thread, non-temporal) variant of the
-// tuning table; B must not exceed the tile of that variant, else the
-// one-thread-per-block kernel runs.
+// Values per thread actually launched: the tuned one, raised to the smallest
+// supported one whose tile (256 x VPT values) holds a whole exponent block.
+// 2 values per thread exists for float64 with U >= 16 only (one 16-byte load
+// per lane).  0 = no tile holds the block (the one-thread-per-block kernels).
+template<typename T, int U>
+int
+effective_vpt(int vpt, std::uint64_t B)
+{
+  if (vpt == 2 && !(sizeof(T) == 8 && U >= 16)) {
+    vpt = 4;
+  }
+  while (vpt <= 16 && static_cast<std::uint64_t>(WG) * vpt < B) {
+    vpt *= 2;
+  }
+  return vpt <= 16 ? vpt : 0;
+}
+
+// Tiled launch with the variant of the tuning table (values per thread,
+// non-temporal hint, tile order, staged decompress stores); B beyond the
+// largest tile, or unaligned pointers, take the one-thread-per-block kernels.
 template<typename T, int U>
 hipError_t
 run_compress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
-  if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
-    switch (v.vpt * 2 + v.nt) {
-      case 8:
-        return launch_compress_tiled<T, U, 4, false>(in, out, L, lb, v.linear, s);
-      case 9:
-        return launch_compress_tiled<T, U, 4, true>(in, out, L, lb, v.linear, s);
-      case 16:
-        return launch_compress_tiled<T, U, 8, false>(in, out, L, lb, v.linear, s);
-      case 17:
-        return launch_compress_tiled<T, U, 8, true>(in, out, L, lb, v.linear, s);
-      case 32:
-        return launch_compress_tiled<T, U, 16, false>(in, out, L, lb, v.linear, s);
-      default:
-        return launch_compress_tiled<T, U, 16, true>(in, out, L, lb, v.linear, s);
+  const int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B) : 0;
+  if (vpt > 0) {
+#define FRSZ2_CL(V, N) return launch_compress_tiled<T, U, V, N>(in, out, L, lb, v.linear, s)
+    if constexpr (sizeof(T) == 8 && U >= 16) {
+      if (vpt == 2) {
+        if (v.nt) {
+          FRSZ2_CL(2, true);
+        }
+        FRSZ2_CL(2, false);
+      }
     }
+    switch (vpt * 2 + v.nt) {
+      case 8: FRSZ2_CL(4, false);
+      case 9: FRSZ2_CL(4, true);
+      case 16: FRSZ2_CL(8, false);
+      case 17: FRSZ2_CL(8, true);
+      case 32: FRSZ2_CL(16, false);
+      default: FRSZ2_CL(16, true);
+    }
+#undef FRSZ2_CL
   }
   return launch_compress_generic<T, U>(in, out, L, s);
 }
@@ -263,9 +285,20 @@ hipError_t
 run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
-  if (tiled && static_cast<std::uint64_t>(WG) * v.vpt >= L.B) {
+  const int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B) : 0;
+  if (vpt > 0) {
 #define FRSZ2_DL(V, N, S) return launch_decompress_tiled<T, U, V, N, S>(in, out, L, lb, v.linear, s)
-    switch (v.vpt * 4 + v.stage * 2 + v.nt) {
+    if constexpr (sizeof(T) == 8 && U >= 16) {
+      if (vpt == 2) {
+        switch (v.stage * 2 + v.nt) {
+          case 0: FRSZ2_DL(2, false, false);
+          case 1: FRSZ2_DL(2, true, false);
+          case 2: FRSZ2_DL(2, false, true);
+          default: FRSZ2_DL(2, true, true);
+        }
+      }
+    }
+    switch (vpt * 4 + v.stage * 2 + v.nt) {
       case 16: FRSZ2_DL(4, false, false);
       case 17: FRSZ2_DL(4, true, false);
       case 18: FRSZ2_DL(4, false, true);

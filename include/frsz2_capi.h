@@ -117,8 +117,9 @@ int frsz2_gather(int dtype,
                  frsz2_stream_t stream);
 
 /* Kernel variant of the tiled plain kernels for (op, dtype, bits): op 0 =
- * compress, 1 = decompress; values_per_thread 4, 8 or 16 (a tile is 256 x
- * that many values, so it also bounds the largest B the tiled path takes);
+ * compress, 1 = decompress; values_per_thread 2 (float64 with bits > 8
+ * only; otherwise taken as 4), 4, 8 or 16: a tile is 256 x that many values
+ * and is raised to the next size that holds a whole exponent block;
  * flags: FRSZ2_TUNE_NONTEMPORAL puts the non-temporal hint on the streaming
  * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) writes each tile's output
  * through LDS with whole-cache-line stores, FRSZ2_TUNE_LINEAR_ORDER takes
