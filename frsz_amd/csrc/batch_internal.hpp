@@ -34,7 +34,7 @@ static_assert(sizeof(bjob) == 96, "descriptor layout");
 
 struct batch_shape
 {
-  int tile_shape;      // 0 narrow, 1 balanced, 2 wide
+  int tile_shape;      // 0 narrow, 1 balanced, 2 wide, 3 grouped (one launch per class)
   int workgroups;      // persistent grid
   int tiles_per_claim; // 0 = static round-robin
   int ops;             // directions present (1 compress, 2 decompress, 4 gather); 0 = any
@@ -51,8 +51,9 @@ std::uint64_t build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_s
                                 std::vector<char>& tiled);
 
 // Counter reset (when claiming) + persistent kernel over ndesc descriptors in
-// device memory, on stream s.
-hipError_t launch_persistent(const bjob* d_desc, int ndesc, std::uint64_t total_tiles,
+// device memory (d_desc; h_desc is the host copy), on stream s.  Tile shape 3
+// launches one grouped kernel per class instead.
+hipError_t launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
                              unsigned long long* d_counter, const batch_shape& shape, hipStream_t s);
 
 // One job as a plain single-array launch on s.

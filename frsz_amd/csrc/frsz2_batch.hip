@@ -24,6 +24,7 @@
 #include "frsz2_internal.hpp"
 #include "tile.hpp"
 #include "batch_internal.hpp"
+#include "tuning_defaults.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -82,6 +83,19 @@ gather_tile(const bjob& J, std::uint64_t t)
   }
 }
 
+// (op, dtype, U index, packed) of a class
+__host__ __device__ constexpr int cls_op(int cls) { return cls >= 28 ? 2 : cls / 14; }
+__host__ __device__ constexpr int cls_dtype(int cls) { return cls >= 28 ? (cls < 31 ? 0 : 1) : ((cls % 14) < 6 ? 0 : 1); }
+__host__ __device__ constexpr int cls_ui(int cls)
+{
+  return cls >= 28 ? (cls < 31 ? cls - 28 : cls - 31) : ((cls % 14) < 6 ? (cls % 14) / 2 : ((cls % 14) - 6) / 2);
+}
+__host__ __device__ constexpr int cls_packed(int cls) { return cls >= 28 ? 0 : (cls % 14) % 2; }
+__host__ __device__ constexpr int cls_default(int cls)
+{
+  return default_variant(cls_op(cls) == 2 ? 1 : cls_op(cls), cls_dtype(cls), cls_ui(cls), cls_packed(cls));
+}
+
 // Per-class tile shape, three settings:
 //   0 narrow:   4 values per thread for every class (smallest tiles, finest
 //               load balance, fewest registers), float64 decompress staged
@@ -89,11 +103,17 @@ gather_tile(const bjob& J, std::uint64_t t)
 //               LDS output stage
 //   2 wide:     the shapes measured fastest for the plain kernels (up to 16
 //               values per thread, staged decompress stores)
+//   3 grouped:  not one persistent kernel but one launch per class present,
+//               each with that class's default plain-kernel variant and its
+//               own register allocation (launch_grouped)
 // In one kernel every class shares the register allocation of the widest,
 // so wider shapes trade occupancy for bytes in flight; the autotuner picks.
 __host__ __device__ constexpr int
 class_vpt(int cls, int shape)
 {
+  if (shape == 3) {
+    return venc_vpt(cls_default(cls)); // grouped: the plain kernels' default variant
+  }
   const int op = cls / 14;
   const int slot = cls % 14;
   const bool f32 = slot < 6;
@@ -117,6 +137,9 @@ class_vpt(int cls, int shape)
 __host__ __device__ constexpr bool
 class_stage(int cls, int shape)
 {
+  if (shape == 3) {
+    return cls_op(cls) == 1 && venc_stage(cls_default(cls)) != 0;
+  }
   const int slot = cls % 14;
   const bool f32 = slot < 6;
   const bool packed = (slot % 2) == 1;
@@ -298,6 +321,53 @@ k_reset_counter(unsigned long long* counter)
   *counter = 0ull;
 }
 
+// Grouped mode: one workgroup per tile of the jobs of one class (jobs[0..njobs)
+// are that class's descriptors, tiles tile_base + blockIdx.x).
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP>
+__global__ __launch_bounds__(WG) void k_grouped(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base)
+{
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
+  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT>() : 1];
+  __shared__ std::uint32_t wred[WG / tile::WAVE];
+  const std::uint64_t g = tile_base + blockIdx.x;
+  const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, 0));
+  const bjob J = jobs[j];
+  const std::uint64_t t = g - J.tile0;
+  if constexpr (OP == 2) {
+    gather_tile<T, U>(J, t);
+  } else {
+    const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb < 0 ? 0 : J.lb, 0 };
+    if constexpr (OP == 0) {
+      auto* o8 = static_cast<std::uint8_t*>(J.out);
+      tile::compress_tile<T, U, VPT, PACKED, NT>(static_cast<const typename fmt<T>::uw*>(J.in), o8, p, t, lds,
+                                                 wred);
+      if (t == 0) {
+        tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);
+      }
+    } else {
+      tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE>(static_cast<const std::uint8_t*>(J.in),
+                                                          static_cast<typename fmt<T>::uw*>(J.out), p, t, lds,
+                                                          stage);
+    }
+  }
+}
+
+template<int CLS>
+void
+launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, hipStream_t s)
+{
+  using T = std::conditional_t<cls_dtype(CLS) == 0, float, double>;
+  constexpr int U = 8 << cls_ui(CLS);
+  constexpr int V = cls_default(CLS);
+  constexpr int OP = cls_op(CLS);
+  constexpr int VPT = OP == 2 ? 4 : venc_vpt(V);
+  constexpr bool PACKED = cls_packed(CLS) != 0;
+  constexpr bool NT = venc_nt(V) != 0;
+  constexpr bool STAGE = OP == 1 && venc_stage(V) != 0;
+  hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP>), dim3(static_cast<unsigned>(nt)), dim3(WG), 0,
+                     s, d, nd, t0);
+}
+
 int
 ilog2_exact(std::uint64_t B)
 {
@@ -315,6 +385,45 @@ bool
 aligned16(const void* p)
 {
   return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0;
+}
+
+// grouped mode: one k_grouped launch per run of same-class descriptors
+// (descriptors are built in class order)
+hipError_t
+launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, hipStream_t s)
+{
+  int d0 = 0;
+  while (d0 < nd) {
+    const int cls = h_desc[d0].cls;
+    int d1 = d0 + 1;
+    while (d1 < nd && h_desc[d1].cls == cls) {
+      ++d1;
+    }
+    const std::uint64_t t0 = h_desc[d0].tile0;
+    const std::uint64_t t1 = d1 < nd ? h_desc[d1].tile0 : total_tiles;
+    if (t1 > t0) {
+#define FRSZ2_G(K)                                                                                           \
+  case K:                                                                                                    \
+    launch_class<K>(d_desc + d0, d1 - d0, t0, t1 - t0, s);                                                   \
+    break
+      switch (cls) {
+        FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
+        FRSZ2_G(7); FRSZ2_G(8); FRSZ2_G(9); FRSZ2_G(10); FRSZ2_G(11); FRSZ2_G(12); FRSZ2_G(13);
+        FRSZ2_G(14); FRSZ2_G(15); FRSZ2_G(16); FRSZ2_G(17); FRSZ2_G(18); FRSZ2_G(19); FRSZ2_G(20);
+        FRSZ2_G(21); FRSZ2_G(22); FRSZ2_G(23); FRSZ2_G(24); FRSZ2_G(25); FRSZ2_G(26); FRSZ2_G(27);
+        FRSZ2_G(28); FRSZ2_G(29); FRSZ2_G(30); FRSZ2_G(31); FRSZ2_G(32); FRSZ2_G(33); FRSZ2_G(34);
+        default:
+          break;
+      }
+#undef FRSZ2_G
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        return e;
+      }
+    }
+    d0 = d1;
+  }
+  return hipSuccess;
 }
 
 } // namespace
@@ -411,11 +520,14 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
 }
 
 hipError_t
-launch_persistent(const bjob* d_desc, int ndesc, std::uint64_t total_tiles, unsigned long long* d_counter,
-                  const batch_shape& shape, hipStream_t s)
+launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
+                  unsigned long long* d_counter, const batch_shape& shape, hipStream_t s)
 {
   if (ndesc == 0 || total_tiles == 0) {
     return hipSuccess;
+  }
+  if (shape.tile_shape == 3) {
+    return launch_grouped(d_desc, h_desc, ndesc, total_tiles, s);
   }
   const int chunk = shape.tiles_per_claim;
   std::uint64_t grid = static_cast<std::uint64_t>(shape.workgroups);
@@ -529,7 +641,8 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
     return FRSZ2_OK;
   }
   const batch_shape shape{ P->cfg.tile_shape, P->cfg.workgroups, P->cfg.tiles_per_claim, P->ops };
-  const hipError_t e = launch_persistent(P->d_desc, P->ndesc, P->total_tiles, P->d_counter, shape, s);
+  const hipError_t e =
+    launch_persistent(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, P->d_counter, shape, s);
   if (e != hipSuccess) {
     return hip_status(e, "batch launch");
   }
@@ -594,7 +707,7 @@ int
 apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
 {
   frsz2_batch_config n = c;
-  if (n.tile_shape < 0 || n.tile_shape > 2) {
+  if (n.tile_shape < 0 || n.tile_shape > 3) {
     n.tile_shape = 1;
   }
   if (n.workgroups <= 0) {
@@ -734,13 +847,16 @@ frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, fr
   }
   frsz2_batch_config best = plan->cfg;
   float best_ms = -1.0f;
-  const int shapes[3] = { 0, 1, 2 };
+  const int shapes[4] = { 0, 1, 2, 3 };
   const int wg_per_cu[4] = { 2, 4, 8, 16 };
   const int chunks[4] = { 0, 1, 4, 16 };
   int rc = FRSZ2_OK;
-  for (int vi = 0; vi < 3 && rc == FRSZ2_OK; ++vi) {
+  for (int vi = 0; vi < 4 && rc == FRSZ2_OK; ++vi) {
     for (int wi = 0; wi < 4 && rc == FRSZ2_OK; ++wi) {
       for (int ci = 0; ci < 4 && rc == FRSZ2_OK; ++ci) {
+        if (shapes[vi] == 3 && (wi > 0 || ci > 0)) {
+          continue; // grouped launches have no grid or claim setting
+        }
         frsz2_batch_config c{};
         c.tile_shape = shapes[vi];
         c.workgroups = wg_per_cu[wi] * plan->cus;

@@ -8,6 +8,7 @@
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
+#include "tuning_defaults.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -34,48 +35,15 @@ u_index(int bits)
   return U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3;
 }
 
-// encoded table entry
-constexpr int
-venc(int vpt, int stage, int nt, int linear = 0)
-{
-  return vpt * 8 + linear * 4 + stage * 2 + nt;
-}
-
-// Defaults measured on MI355X with the plain kernels (1 GiB float64 field,
-// 10^8-value float32 arrays): [op][dtype][U index][packed].
+// Defaults measured on MI355X (tuning_defaults.hpp): [op][dtype][U index][packed].
 void
 fill_variants()
 {
-  // venc(values per thread, staged stores, non-temporal, linear tile order)
-  static const int table[2][2][4][2] = {
-    // compress
-    { // float32: U8, U16, U32 (U64 unused) x {aligned, packed}
-      { { venc(8, 0, 0, 1), venc(8, 0, 0, 0) },
-        { venc(8, 0, 1, 1), venc(8, 0, 0, 0) },
-        { venc(4, 0, 1, 1), venc(8, 0, 0, 0) },
-        { venc(4, 0, 1, 1), venc(4, 0, 1, 1) } },
-      // float64
-      { { venc(4, 0, 1, 1), venc(4, 0, 1, 0) },
-        { venc(4, 0, 1, 0), venc(4, 0, 1, 0) },
-        { venc(4, 0, 1, 1), venc(4, 0, 1, 0) },
-        { venc(4, 0, 1, 1), venc(4, 0, 1, 0) } } },
-    // decompress
-    { // float32
-      { { venc(16, 1, 1, 1), venc(8, 0, 1, 1) },
-        { venc(16, 1, 1, 1), venc(8, 0, 1, 1) },
-        { venc(8, 0, 1, 1), venc(8, 0, 1, 1) },
-        { venc(4, 0, 1, 1), venc(4, 0, 1, 1) } },
-      // float64
-      { { venc(8, 1, 1, 1), venc(4, 0, 1, 1) },
-        { venc(8, 1, 1, 1), venc(4, 0, 1, 1) },
-        { venc(8, 1, 1, 1), venc(4, 0, 1, 1) },
-        { venc(8, 1, 1, 1), venc(4, 0, 1, 1) } } }
-  };
   for (int op = 0; op < 2; ++op)
     for (int d = 0; d < 2; ++d)
       for (int u = 0; u < 4; ++u)
         for (int pk = 0; pk < 2; ++pk)
-          g_variant[op][d][u][pk].store(table[op][d][u][pk], std::memory_order_relaxed);
+          g_variant[op][d][u][pk].store(default_variant(op, d, u, pk), std::memory_order_relaxed);
 }
 
 void

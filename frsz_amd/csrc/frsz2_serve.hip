@@ -371,7 +371,7 @@ launch_slot(frsz2_server* S, slot& sl)
       ops |= 1 << j.op;
     }
     const batch_shape shape{ S->cfg.tile_shape, S->cfg.workgroups, S->cfg.tiles_per_claim, ops };
-    e = launch_persistent(sl.d_desc, nd, tiles, sl.d_counter, shape, sl.stream);
+    e = launch_persistent(sl.d_desc, sl.desc.data(), nd, tiles, sl.d_counter, shape, sl.stream);
     if (e != hipSuccess) {
       return hip_status(e, "batch launch");
     }
@@ -599,7 +599,7 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
   if (S->cfg.max_batch <= 0) {
     S->cfg.max_batch = 1024;
   }
-  if (S->cfg.tile_shape < 0 || S->cfg.tile_shape > 2) {
+  if (S->cfg.tile_shape < 0 || S->cfg.tile_shape > 3) {
     S->cfg.tile_shape = 0;
   }
   if (S->cfg.workgroups <= 0) {

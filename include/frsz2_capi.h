@@ -171,7 +171,10 @@ typedef struct frsz2_batch_config
   int tile_shape;        /* 0 narrow: 4 values per thread for every class;
                             1 balanced: up to 8, float64 decompress staged;
                             2 wide: the plain kernels' fastest shapes (up to
-                            16 values per thread, staged decompress stores) */
+                            16 values per thread, staged decompress stores);
+                            3 grouped: instead of one persistent kernel, one
+                            launch per kernel class present (same graph), each
+                            with that class's plain-kernel default variant */
   int tiles_per_claim;   /* tiles per atomic claim; 0 = static round-robin */
   int use_graph;         /* 1 = launch through a captured hipGraph */
   int plain_launches;    /* 1 = no persistent kernel: one plain frsz2_compress /

@@ -53,10 +53,13 @@ CONFIGS = [None,
            {"workgroups": 64, "tile_shape": 0, "tiles_per_claim": 0, "use_graph": 1},
            {"workgroups": 1024, "tile_shape": 2, "tiles_per_claim": 1, "use_graph": 0},
            {"workgroups": 512, "tile_shape": 2, "tiles_per_claim": 16, "use_graph": 1},
-           {"plain_launches": 1, "use_graph": 1}]
+           {"plain_launches": 1, "use_graph": 1},
+           {"tile_shape": 3, "use_graph": 1},
+           {"tile_shape": 1, "workgroups": 700, "tiles_per_claim": 3, "use_graph": 1}]
 
 
-@pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "static", "nograph", "chunk16", "plain-graph"])
+@pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "static", "nograph", "chunk16", "plain-graph", "grouped",
+                                             "balanced-odd"])
 def test_batch_matches_plain(fz, cfg):
     fz_mod, batch = fz
     specs = make_arrays(1)
