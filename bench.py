@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--no-copy", action="store_true", help="skip the 1 GiB device-copy calibration")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-edge", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
@@ -137,6 +139,141 @@ def bench_batched(count, steps, seed=4, target_input_bytes=None):
     del xs, streams, outs
     torch.cuda.empty_cache()
     return res
+
+
+def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
+    """float32 lognormal magnitudes (exponents over about +-30), 10% all-zero
+    64-value runs, 1% subnormals, random signs, n not a multiple of any block
+    size: compress / decompress GB/s for bits {8,12,16,21,32} x B {16,32,64},
+    and a uniform random-access decode of 10^7 single values (bits 16, B 32).
+    Every output is compared with the first launch's (determinism)."""
+    import numpy as np
+    import torch
+
+    import frsz_amd
+    from frsz_amd.workloads import lognormal_edge
+
+    x = torch.from_numpy(lognormal_edge(n, seed=3)).cuda()
+    out = {"n": n, "cases": []}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    y = torch.empty_like(x)
+    for bits in (8, 12, 16, 21, 32):
+        for B in (16, 32, 64):
+            size = frsz_amd.compressed_size(torch.float32, bits, B, n)
+            s = torch.empty(size, dtype=torch.uint8, device="cuda")
+            frsz_amd.compress_into(x, s, bits, B)
+            first = s.clone()
+            tc = timed(lambda: frsz_amd.compress_into(x, s, bits, B))
+            same = torch.equal(s, first)
+            td = timed(lambda: frsz_amd.decompress_into(s, y, bits, B))
+            alg = n * 4 + size
+            out["cases"].append({"bits": bits, "block": B, "stream_bytes": size,
+                                 "compress_gbs": round(alg / tc / 1e9, 1), "decompress_gbs": round(alg / td / 1e9, 1),
+                                 "deterministic": bool(same)})
+            if (bits, B) == (16, 32):
+                g = torch.from_numpy(np.random.default_rng(7).integers(0, n, gathers).astype(np.int64)).cuda()
+                go = torch.empty(gathers, dtype=torch.float32, device="cuda")
+                from frsz_amd import _capi
+
+                def gat():
+                    _capi.check(_capi.lib.frsz2_gather(0, bits, B, s.data_ptr(), s.numel(), n, g.data_ptr(),
+                                                       gathers, go.data_ptr(), torch.cuda.current_stream().cuda_stream))
+
+                tg = timed(gat)
+                ref = y[g]
+                U = 16
+                out["gather"] = {"indices": gathers, "bits": bits, "block": B, "us": round(tg * 1e6, 1),
+                                 "melem_s": round(gathers / tg / 1e6, 1),
+                                 "algorithmic_gbs": round(gathers * (8 + 4 + U // 8 + U // 8) / tg / 1e9, 1),
+                                 "bound": "transaction (random 64-byte touches), not bandwidth",
+                                 "matches_decompress": bool(torch.equal(go.view(torch.int32), ref.view(torch.int32)))}
+            del s, first
+    del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_host(x_host, bits, block, want_stream, want_y, reps=3):
+    """Host-resident data (the plugin's host-in / host-out meaning): the same
+    field from pageable host memory to a host stream and back through the
+    pipelined frsz2_compress_host / frsz2_decompress_host, beside (a) the
+    unpipelined sequence a caller would otherwise write (torch pageable H2D,
+    device compress, D2H) and (b) the raw PCIe rates of pinned 1 GiB copies.
+    Rates are algorithmic bytes (values + stream) per second, PCIe included."""
+    import numpy as np
+    import torch
+
+    import frsz_amd
+    from frsz_amd import _capi
+
+    n = x_host.size
+    size = frsz_amd.compressed_size(torch.float64, bits, block, n)
+    alg = n * 8 + size
+    s_host = np.empty(size, np.uint8)
+    y_host = np.empty_like(x_host)
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    tc = best(lambda: _capi.check(_capi.lib.frsz2_compress_host(1, bits, block, x_host.ctypes.data, n,
+                                                                 s_host.ctypes.data, size)))
+    td = best(lambda: _capi.check(_capi.lib.frsz2_decompress_host(1, bits, block, s_host.ctypes.data, size, n,
+                                                                   y_host.ctypes.data)))
+    xt = torch.from_numpy(x_host)
+    st = torch.from_numpy(s_host)
+
+    def naive_c():
+        xd = xt.cuda()
+        sd = frsz_amd.compress(xd, bits, block)
+        sd.cpu()
+        torch.cuda.synchronize()
+
+    def naive_d():
+        sd = st.cuda()
+        yd = frsz_amd.decompress(sd, n, torch.float64, bits, block)
+        yd.cpu()
+        torch.cuda.synchronize()
+
+    tnc, tnd = best(naive_c), best(naive_d)
+    pin = torch.empty(1 << 27, dtype=torch.float64, pin_memory=True)
+    dev = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+
+    def h2d():
+        dev.copy_(pin, non_blocking=True)
+        torch.cuda.synchronize()
+
+    def d2h():
+        pin.copy_(dev, non_blocking=True)
+        torch.cuda.synchronize()
+
+    th, tdh = best(h2d), best(d2h)
+    same = bool(np.array_equal(s_host, want_stream) and np.array_equal(y_host.view(np.uint64), want_y.view(np.uint64)))
+    del pin, dev
+    torch.cuda.empty_cache()
+    return {"workload": f"float64 field {n} values, bits {bits}, B {block}, pageable host memory",
+            "compress_gbs": round(alg / tc / 1e9, 2), "decompress_gbs": round(alg / td / 1e9, 2),
+            "unpipelined_compress_gbs": round(alg / tnc / 1e9, 2),
+            "unpipelined_decompress_gbs": round(alg / tnd / 1e9, 2),
+            "pcie_pinned_h2d_gbs": round((1 << 30) / th / 1e9, 2),
+            "pcie_pinned_d2h_gbs": round((1 << 30) / tdh / 1e9, 2),
+            "bit_exact_vs_device_path": same}
 
 
 def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499):
@@ -449,6 +586,11 @@ def main():
         line["copy_calibration"] = {"bytes_read": a.numel() * 8, "bytes_written": a.numel() * 8,
                                     "us": round(tcopy * 1e6, 1), "gbs": round(2 * a.numel() * 8 / tcopy / 1e9, 1)}
         del a, b
+    if not args.no_host:
+        print("bench: host-resident path", file=sys.stderr, flush=True)
+        if x_host is None:
+            x_host = x.cpu().numpy()
+        line["host_path"] = bench_host(x_host, bits, block, stream.cpu().numpy(), y.cpu().numpy())
     del x, y, stream
     torch.cuda.empty_cache()
     if not args.no_batched:
@@ -456,6 +598,9 @@ def main():
         line["batched"] = bench_batched(args.batched_arrays, args.batched_steps)
         print("bench: batched workload scaled to 4 GiB of input", file=sys.stderr, flush=True)
         line["batched_4gib"] = bench_batched(args.batched_arrays, args.batched_steps, target_input_bytes=1 << 32)
+    if not args.no_edge:
+        print("bench: edge-case sweep", file=sys.stderr, flush=True)
+        line["edge"] = bench_edge()
     if not args.no_serve:
         line["serve"] = bench_serve(args.serve_seconds, args.serve_rate, args.serve_budget_gib)
     if not args.no_cpu:

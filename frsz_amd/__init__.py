@@ -9,9 +9,11 @@ from ._capi import FLOAT32, FLOAT64, Frsz2Error, version
 from .codec import (
     Frsz2Compressor,
     compress,
+    compress_blocks,
     compress_into,
     compressed_size,
     decompress,
+    decompress_blocks,
     decompress_into,
     dtype_code,
     gather,
@@ -23,9 +25,11 @@ __all__ = [
     "Frsz2Error",
     "Frsz2Compressor",
     "compress",
+    "compress_blocks",
     "compress_into",
     "compressed_size",
     "decompress",
+    "decompress_blocks",
     "decompress_into",
     "dtype_code",
     "gather",

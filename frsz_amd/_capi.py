@@ -73,6 +73,14 @@ lib.frsz2_decompress_host.restype = _int
 lib.frsz2_decompress_host.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp]
 lib.frsz2_gather_host.restype = _int
 lib.frsz2_gather_host.argtypes = [_int, _int, _u64, _vp, _u64, _u64, _vp, _u64, _vp]
+lib.frsz2_compress_blocks.restype = _int
+lib.frsz2_compress_blocks.argtypes = [_int, _int, _u64, _u64, _u64, _vp, _u64, _vp, _u64, _vp]
+lib.frsz2_decompress_blocks.restype = _int
+lib.frsz2_decompress_blocks.argtypes = [_int, _int, _u64, _u64, _u64, _vp, _u64, _u64, _vp, _vp]
+lib.frsz2_compress_blocks_host.restype = _int
+lib.frsz2_compress_blocks_host.argtypes = [_int, _int, _u64, _u64, _u64, _vp, _u64, _vp, _u64]
+lib.frsz2_decompress_blocks_host.restype = _int
+lib.frsz2_decompress_blocks_host.argtypes = [_int, _int, _u64, _u64, _u64, _vp, _u64, _u64, _vp]
 
 
 def check(rc: int) -> None:

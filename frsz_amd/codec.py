@@ -32,7 +32,9 @@ __all__ = [
     "compressed_size",
     "compress",
     "compress_into",
+    "compress_blocks",
     "decompress",
+    "decompress_blocks",
     "decompress_into",
     "gather",
     "dtype_code",
@@ -106,6 +108,35 @@ def gather(stream: torch.Tensor, n: int, dtype: torch.dtype, bits: int, block: i
     out = torch.empty(idx.numel(), dtype=dtype, device=s.device)
     check(_capi.lib.frsz2_gather(dtype_code(dtype), bits, block, s.data_ptr(), s.numel(), n, idx.data_ptr(),
                                  idx.numel(), out.data_ptr(), _stream()))
+    return out
+
+
+def compress_blocks(values: torch.Tensor, stream: torch.Tensor, n: int, first_block: int, bits: int,
+                    block: int) -> torch.Tensor:
+    """Compress ``values`` -- values ``[first_block*block, +values.numel())``
+    of an ``n``-value array -- into their exponent slots and payload blocks
+    of that array's ``stream`` (compress_cpu_block, include/frsz2.hpp:1108-1179,
+    over a run of blocks).  Runs over disjoint blocks together give the bytes
+    of ``compress``.  Returns ``stream``."""
+    v = _flat(values)
+    s = _flat(stream)
+    if s.dtype != torch.uint8:
+        raise Frsz2Error(_capi.EARG, "stream must be a uint8 GPU tensor")
+    check(_capi.lib.frsz2_compress_blocks(dtype_code(v.dtype), bits, block, n, first_block, v.data_ptr(), v.numel(),
+                                          s.data_ptr(), s.numel(), _stream()))
+    return stream
+
+
+def decompress_blocks(stream: torch.Tensor, out: torch.Tensor, n: int, first_block: int, bits: int,
+                      block: int) -> torch.Tensor:
+    """Decode values ``[first_block*block, +out.numel())`` of the ``n``-value
+    array whose stream is ``stream`` into ``out`` (decompress_cpu_block,
+    include/frsz2.hpp:1217-1240, over a run of blocks).  Returns ``out``."""
+    if not out.is_cuda or not out.is_contiguous():
+        raise Frsz2Error(_capi.EARG, "output must be a contiguous GPU tensor")
+    s = _flat(stream)
+    check(_capi.lib.frsz2_decompress_blocks(dtype_code(out.dtype), bits, block, n, first_block, s.data_ptr(),
+                                            s.numel(), out.numel(), out.data_ptr(), _stream()))
     return out
 
 

@@ -1,10 +1,11 @@
-// Error state and the host-memory entry points of libfrsz2_hip.so.
+// Error state, the kernel-variant table and the host-memory gather of
+// libfrsz2_hip.so (the pipelined host compress / decompress live in
+// frsz2_pipeline.hip).
 //
-// The host variants keep the host-in / host-out meaning of the reference's
-// CPU codec (include/frsz2.hpp:1181-1249) and LibPressio plugin
-// (src/frsz2.cc:152-186): they copy the input to device memory, run the same
-// HIP kernels as the device entry points on a private stream, and copy the
-// result back before returning.
+// frsz2_gather_host keeps the host-in / host-out meaning of the reference's
+// decompress_cpu_element (include/frsz2.hpp:1199-1214): it copies the stream
+// and the indices to device memory, runs the gather kernel on a private
+// stream, and copies the values back before returning.
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
@@ -157,78 +158,6 @@ frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags)
   *values_per_thread = v.vpt;
   *flags = (v.nt ? FRSZ2_TUNE_NONTEMPORAL : 0) | (v.stage ? FRSZ2_TUNE_STAGED_STORES : 0) |
            (v.linear ? FRSZ2_TUNE_LINEAR_ORDER : 0);
-  return FRSZ2_OK;
-}
-
-int
-frsz2_compress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t n, void* h_out,
-                    uint64_t out_capacity)
-{
-  using namespace frsz2;
-  clear_error();
-  int rc = check_params(dtype, bits, block);
-  if (rc) {
-    return rc;
-  }
-  const layout L = layout::make(bits, block, n);
-  const std::uint64_t total = L.total_bytes();
-  if (out_capacity < total) {
-    return set_error(FRSZ2_ECAPACITY,
-                     "output capacity " + std::to_string(out_capacity) + " < stream size " + std::to_string(total));
-  }
-  if (!h_out || (n > 0 && !h_in)) {
-    return set_error(FRSZ2_EARG, "null pointer");
-  }
-  const std::uint64_t in_bytes = n * (dtype == FRSZ2_FLOAT32 ? 4u : 8u);
-  staging st;
-  FRSZ2_HIP_TRY(st.init(), "hipStreamCreate");
-  FRSZ2_HIP_TRY(st.alloc(0, in_bytes), "hipMalloc");
-  FRSZ2_HIP_TRY(st.alloc(1, total), "hipMalloc");
-  if (in_bytes) {
-    FRSZ2_HIP_TRY(hipMemcpyAsync(st.bufs[0], h_in, in_bytes, hipMemcpyHostToDevice, st.s), "hipMemcpyAsync");
-  }
-  rc = compress_device(dtype, bits, block, st.bufs[0], n, st.bufs[1], total, st.s);
-  if (rc) {
-    return rc;
-  }
-  FRSZ2_HIP_TRY(hipMemcpyAsync(h_out, st.bufs[1], total, hipMemcpyDeviceToHost, st.s), "hipMemcpyAsync");
-  FRSZ2_HIP_TRY(hipStreamSynchronize(st.s), "hipStreamSynchronize");
-  return FRSZ2_OK;
-}
-
-int
-frsz2_decompress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
-                      void* h_out)
-{
-  using namespace frsz2;
-  clear_error();
-  int rc = check_params(dtype, bits, block);
-  if (rc) {
-    return rc;
-  }
-  const layout L = layout::make(bits, block, n);
-  const std::uint64_t total = L.total_bytes();
-  if (in_bytes < total) {
-    return set_error(FRSZ2_ECAPACITY,
-                     "input stream of " + std::to_string(in_bytes) + " bytes < stream size " + std::to_string(total));
-  }
-  if (!h_in || (n > 0 && !h_out)) {
-    return set_error(FRSZ2_EARG, "null pointer");
-  }
-  const std::uint64_t out_bytes = n * (dtype == FRSZ2_FLOAT32 ? 4u : 8u);
-  staging st;
-  FRSZ2_HIP_TRY(st.init(), "hipStreamCreate");
-  FRSZ2_HIP_TRY(st.alloc(0, total), "hipMalloc");
-  FRSZ2_HIP_TRY(st.alloc(1, out_bytes), "hipMalloc");
-  FRSZ2_HIP_TRY(hipMemcpyAsync(st.bufs[0], h_in, total, hipMemcpyHostToDevice, st.s), "hipMemcpyAsync");
-  rc = decompress_device(dtype, bits, block, st.bufs[0], total, n, st.bufs[1], st.s);
-  if (rc) {
-    return rc;
-  }
-  if (out_bytes) {
-    FRSZ2_HIP_TRY(hipMemcpyAsync(h_out, st.bufs[1], out_bytes, hipMemcpyDeviceToHost, st.s), "hipMemcpyAsync");
-  }
-  FRSZ2_HIP_TRY(hipStreamSynchronize(st.s), "hipStreamSynchronize");
   return FRSZ2_OK;
 }
 

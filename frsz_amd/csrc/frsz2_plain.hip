@@ -41,7 +41,7 @@ __global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw*
   const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
   tile::compress_tile<T, U, VPT, PACKED, NT>(in, out, p, t, lds, wred);
   if (t == 0) {
-    tile::zero_pad(out, p.nb, U, p.R, static_cast<int>(threadIdx.x), WG);
+    tile::zero_plan_pad<U>(out, p, static_cast<int>(threadIdx.x), WG);
   }
 }
 
@@ -58,51 +58,28 @@ __global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __re
   tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE>(in, out, p, t, lds, stage);
 }
 
-// One thread per exponent block, any B; builds each payload word from the
-// codes that overlap it and writes all CB bytes of the block.
+// One thread per exponent block, any B (encode_block: each payload word is
+// built from the codes that overlap it, all CB bytes of the block written).
+// The first 256 threads clear the zpad exponent-pad bytes after slot nb.
 template<typename T, int U>
-__global__ __launch_bounds__(WG) void k_compress_block(const typename fmt<T>::uw* __restrict__ in,
+__global__ __launch_bounds__(WG) void k_compress_block(const T* __restrict__ in,
                                                        std::uint8_t* __restrict__ out,
-                                                       layout L)
+                                                       layout L,
+                                                       std::uint32_t zpad)
 {
-  using uw = typename fmt<T>::uw;
-  using wt = typename word<U>::type;
   const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
   if (gid < 256) {
-    tile::zero_pad(out, L.nb, U, L.R, static_cast<int>(gid), 256);
+    std::uint8_t* q = out + L.nb * static_cast<std::uint64_t>(U / 8);
+    for (std::uint32_t i = static_cast<std::uint32_t>(gid); i < zpad; i += 256) {
+      q[i] = 0;
+    }
   }
   if (gid >= L.nb) {
     return;
   }
-  const std::uint64_t k = gid;
-  const std::uint64_t first = k * L.B;
+  const std::uint64_t first = gid * L.B;
   const std::uint64_t len = (L.n - first) < L.B ? (L.n - first) : L.B;
-  std::uint32_t m = 0;
-  for (std::uint64_t i = 0; i < len; ++i) {
-    const std::uint32_t key = max_key<T>(in[first + i]);
-    m = m > key ? m : key;
-  }
-  const int E = block_exponent_from_key<T, U>(m);
-  tile::store_exponent<U>(out, k, E);
-  wt* words = reinterpret_cast<wt*>(out + L.R + k * L.CB);
-  const std::uint64_t nwords = L.CB / (U / 8);
-  const std::uint64_t bits = static_cast<std::uint64_t>(L.bits);
-  for (std::uint64_t w = 0; w < nwords; ++w) {
-    const std::uint64_t lo = w * U;
-    const std::uint64_t hi = lo + U;
-    std::uint64_t j0 = lo / bits;
-    std::uint64_t j1 = (hi + bits - 1) / bits;
-    if (j1 > len) {
-      j1 = len;
-    }
-    std::uint64_t acc = 0;
-    for (std::uint64_t j = j0; j < j1; ++j) {
-      const std::uint64_t c = static_cast<std::uint64_t>(encode_value<T>(in[first + j], E, L.bits));
-      const std::int64_t sh = static_cast<std::int64_t>(j * bits) - static_cast<std::int64_t>(lo);
-      acc |= sh >= 0 ? (c << sh) : (c >> (-sh));
-    }
-    words[w] = static_cast<wt>(acc);
-  }
+  encode_block<T, U>(out, L, gid, in + first, len);
 }
 
 template<typename T, int U>
@@ -155,10 +132,11 @@ grid_for(std::uint64_t work)
 
 template<typename T, int U, int VPT, bool NT>
 hipError_t
-launch_compress_tiled(const void* in, void* out, const layout& L, int lb, int linear, hipStream_t s)
+launch_compress_tiled(const void* in, void* out, const layout& L, std::uint32_t zpad, int lb, int linear,
+                      hipStream_t s)
 {
   constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
-  tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0 };
+  tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, zpad };
   std::uint64_t nt = (L.n + TILE - 1) / TILE;
   if (nt == 0) {
     nt = 1;
@@ -180,7 +158,7 @@ hipError_t
 launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, int linear, hipStream_t s)
 {
   constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
-  tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0 };
+  tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0u };
   const std::uint64_t nt = (L.n + TILE - 1) / TILE;
   if (nt == 0) {
     return hipSuccess;
@@ -199,11 +177,11 @@ launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, int 
 
 template<typename T, int U>
 hipError_t
-launch_compress_generic(const void* in, void* out, const layout& L, hipStream_t s)
+launch_compress_generic(const void* in, void* out, const layout& L, std::uint32_t zpad, hipStream_t s)
 {
   std::uint64_t threads = L.nb < 256 ? 256 : L.nb;
-  hipLaunchKernelGGL((k_compress_block<T, U>), dim3(grid_for(threads)), dim3(WG), 0, s,
-                     static_cast<const typename fmt<T>::uw*>(in), static_cast<std::uint8_t*>(out), L);
+  hipLaunchKernelGGL((k_compress_block<T, U>), dim3(grid_for(threads)), dim3(WG), 0, s, static_cast<const T*>(in),
+                     static_cast<std::uint8_t*>(out), L, zpad);
   return hipGetLastError();
 }
 
@@ -253,12 +231,12 @@ effective_vpt(int vpt, std::uint64_t B)
 // largest tile, or unaligned pointers, take the one-thread-per-block kernels.
 template<typename T, int U>
 hipError_t
-run_compress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   const int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B) : 0;
   if (vpt > 0) {
-#define FRSZ2_CL(V, N) return launch_compress_tiled<T, U, V, N>(in, out, L, lb, v.linear, s)
+#define FRSZ2_CL(V, N) return launch_compress_tiled<T, U, V, N>(in, out, L, zpad, lb, v.linear, s)
     if constexpr (sizeof(T) == 8 && U >= 16) {
       if (vpt == 2) {
         if (v.nt) {
@@ -277,7 +255,7 @@ run_compress(const void* in, void* out, const layout& L, bool tiled, int lb, hip
     }
 #undef FRSZ2_CL
   }
-  return launch_compress_generic<T, U>(in, out, L, s);
+  return launch_compress_generic<T, U>(in, out, L, zpad, s);
 }
 
 template<typename T, int U>
@@ -346,9 +324,10 @@ by_type(int dtype, int U, A&&... a)
 template<typename T, int U>
 struct compress_op
 {
-  static hipError_t run(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+  static hipError_t run(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, int lb,
+                        hipStream_t s)
   {
-    return run_compress<T, U>(in, out, L, tiled, lb, s);
+    return run_compress<T, U>(in, out, L, zpad, tiled, lb, s);
   }
 };
 
@@ -406,48 +385,116 @@ hip_status(hipError_t e, const char* what)
   return FRSZ2_OK;
 }
 
+run_plan
+make_run(int bits, std::uint64_t block, std::uint64_t n, std::uint64_t first, std::uint64_t count)
+{
+  const layout G = layout::make(bits, block, n);
+  const std::uint64_t es = static_cast<std::uint64_t>(G.U / 8);
+  run_plan r{};
+  r.L = G;
+  r.L.n = count;
+  r.L.nb = count / block + (count % block != 0);
+  r.exp_off = first * es;
+  r.pay_off = G.R + first * G.CB;
+  r.L.R = r.pay_off - r.exp_off;
+  r.zpad = first + r.L.nb == G.nb ? static_cast<std::uint32_t>(G.R - G.nb * es) : 0u;
+  return r;
+}
+
 int
-compress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t n, void* d_out,
-                std::uint64_t out_capacity, hipStream_t s)
+check_run(int dtype, int bits, std::uint64_t block, std::uint64_t n, std::uint64_t first, std::uint64_t count)
 {
   int rc = check_params(dtype, bits, block);
   if (rc) {
     return rc;
   }
-  const layout L = layout::make(bits, block, n);
-  if (out_capacity < L.total_bytes()) {
-    return set_error(FRSZ2_ECAPACITY, "output capacity " + std::to_string(out_capacity) + " < stream size " +
-                                        std::to_string(L.total_bytes()));
+  const std::uint64_t nb = n / block + (n % block != 0);
+  const std::uint64_t room = first >= nb ? 0 : n - first * block;
+  if (first > nb || count > room) {
+    return set_error(FRSZ2_EARG, "blocks from " + std::to_string(first) + " holding " + std::to_string(count) +
+                                   " values exceed the " + std::to_string(n) + "-value array");
+  }
+  return FRSZ2_OK;
+}
+
+int
+compress_run(int dtype, const void* d_values, void* d_out, const layout& L, std::uint32_t zpad, hipStream_t s)
+{
+  if (L.n == 0 && zpad == 0) {
+    return FRSZ2_OK;
+  }
+  const int lb = ilog2_exact(L.B);
+  const bool tiled = lb >= 0 && aligned(d_values, 16) && aligned(static_cast<const std::uint8_t*>(d_out) + L.R, 16);
+  return hip_status(by_type<compress_op>(dtype, L.U, d_values, d_out, L, zpad, tiled, lb, s), "compress launch");
+}
+
+int
+decompress_run(int dtype, const void* d_in, void* d_values, const layout& L, hipStream_t s)
+{
+  if (L.n == 0) {
+    return FRSZ2_OK;
+  }
+  const int lb = ilog2_exact(L.B);
+  const bool tiled = lb >= 0 && aligned(d_values, 16) && aligned(static_cast<const std::uint8_t*>(d_in) + L.R, 16);
+  return hip_status(by_type<decompress_op>(dtype, L.U, d_in, d_values, L, tiled, lb, s), "decompress launch");
+}
+
+int
+compress_blocks_device(int dtype, int bits, std::uint64_t block, std::uint64_t n, std::uint64_t first,
+                       const void* d_values, std::uint64_t count, void* d_stream, std::uint64_t stream_bytes,
+                       hipStream_t s)
+{
+  int rc = check_run(dtype, bits, block, n, first, count);
+  if (rc) {
+    return rc;
+  }
+  const std::uint64_t total = layout::make(bits, block, n).total_bytes();
+  if (stream_bytes < total) {
+    return set_error(FRSZ2_ECAPACITY,
+                     "output capacity " + std::to_string(stream_bytes) + " < stream size " + std::to_string(total));
   }
   const std::uintptr_t vs = dtype == FRSZ2_FLOAT32 ? 4 : 8;
-  if (!d_out || (n > 0 && !d_in) || !aligned(d_in, vs) || !aligned(d_out, 8)) {
+  if (!d_stream || (count > 0 && !d_values) || !aligned(d_values, vs) || !aligned(d_stream, 8)) {
     return set_error(FRSZ2_EARG, "null or misaligned pointer");
   }
-  const int lb = ilog2_exact(block);
-  const bool tiled = lb >= 0 && aligned(d_in, 16) && aligned(d_out, 16);
-  return hip_status(by_type<compress_op>(dtype, L.U, d_in, d_out, L, tiled, lb, s), "compress launch");
+  const run_plan r = make_run(bits, block, n, first, count);
+  return compress_run(dtype, d_values, static_cast<std::uint8_t*>(d_stream) + r.exp_off, r.L, r.zpad, s);
+}
+
+int
+decompress_blocks_device(int dtype, int bits, std::uint64_t block, std::uint64_t n, std::uint64_t first,
+                         const void* d_stream, std::uint64_t stream_bytes, std::uint64_t count, void* d_values,
+                         hipStream_t s)
+{
+  int rc = check_run(dtype, bits, block, n, first, count);
+  if (rc) {
+    return rc;
+  }
+  const std::uint64_t total = layout::make(bits, block, n).total_bytes();
+  if (stream_bytes < total) {
+    return set_error(FRSZ2_ECAPACITY,
+                     "input stream of " + std::to_string(stream_bytes) + " bytes < stream size " + std::to_string(total));
+  }
+  const std::uintptr_t vs = dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  if (!d_stream || (count > 0 && !d_values) || !aligned(d_stream, 8) || !aligned(d_values, vs)) {
+    return set_error(FRSZ2_EARG, "null or misaligned pointer");
+  }
+  const run_plan r = make_run(bits, block, n, first, count);
+  return decompress_run(dtype, static_cast<const std::uint8_t*>(d_stream) + r.exp_off, d_values, r.L, s);
+}
+
+int
+compress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t n, void* d_out,
+                std::uint64_t out_capacity, hipStream_t s)
+{
+  return compress_blocks_device(dtype, bits, block, n, 0, d_in, n, d_out, out_capacity, s);
 }
 
 int
 decompress_device(int dtype, int bits, std::uint64_t block, const void* d_in, std::uint64_t in_bytes,
                   std::uint64_t n, void* d_out, hipStream_t s)
 {
-  int rc = check_params(dtype, bits, block);
-  if (rc) {
-    return rc;
-  }
-  const layout L = layout::make(bits, block, n);
-  if (in_bytes < L.total_bytes()) {
-    return set_error(FRSZ2_ECAPACITY, "input stream of " + std::to_string(in_bytes) + " bytes < stream size " +
-                                        std::to_string(L.total_bytes()));
-  }
-  const std::uintptr_t vs = dtype == FRSZ2_FLOAT32 ? 4 : 8;
-  if (!d_in || (n > 0 && !d_out) || !aligned(d_in, 8) || !aligned(d_out, vs)) {
-    return set_error(FRSZ2_EARG, "null or misaligned pointer");
-  }
-  const int lb = ilog2_exact(block);
-  const bool tiled = lb >= 0 && aligned(d_in, 16) && aligned(d_out, 16);
-  return hip_status(by_type<decompress_op>(dtype, L.U, d_in, d_out, L, tiled, lb, s), "decompress launch");
+  return decompress_blocks_device(dtype, bits, block, n, 0, d_in, in_bytes, n, d_out, s);
 }
 
 int
@@ -528,6 +575,25 @@ frsz2_gather(int dtype, int bits, uint64_t block, const void* d_in, uint64_t in_
   frsz2::clear_error();
   return frsz2::gather_device(dtype, bits, block, d_in, in_bytes, n, d_idx, m, d_out,
                               static_cast<hipStream_t>(stream));
+}
+
+int
+frsz2_compress_blocks(int dtype, int bits, uint64_t block, uint64_t n, uint64_t first_block, const void* d_values,
+                      uint64_t count, void* d_stream, uint64_t stream_bytes, frsz2_stream_t stream)
+{
+  frsz2::clear_error();
+  return frsz2::compress_blocks_device(dtype, bits, block, n, first_block, d_values, count, d_stream, stream_bytes,
+                                       static_cast<hipStream_t>(stream));
+}
+
+int
+frsz2_decompress_blocks(int dtype, int bits, uint64_t block, uint64_t n, uint64_t first_block,
+                        const void* d_stream, uint64_t stream_bytes, uint64_t count, void* d_values,
+                        frsz2_stream_t stream)
+{
+  frsz2::clear_error();
+  return frsz2::decompress_blocks_device(dtype, bits, block, n, first_block, d_stream, stream_bytes, count,
+                                         d_values, static_cast<hipStream_t>(stream));
 }
 
 } // extern "C"

@@ -41,7 +41,10 @@ struct tplan
   std::uint32_t CB;
   int bits;
   int lb; // B = 1 << lb
-  int pad_;
+  // exponent-pad bytes to clear after exponent slot nb (by the first tile):
+  // the whole pad for a stream, 0 for a run of blocks that does not end at
+  // the stream's last block
+  std::uint32_t zpad;
 };
 
 typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -223,8 +226,7 @@ template<int U>
 __device__ __forceinline__ void
 store_exponent(std::uint8_t* __restrict__ out, std::uint64_t k, int e)
 {
-  using wt = typename word<U>::type;
-  reinterpret_cast<wt*>(out)[k] = static_cast<wt>(static_cast<std::int64_t>(e));
+  store_block_exponent<U>(out, k, e);
 }
 
 // max over the G lanes of an aligned lane group (G a power of two); for
@@ -646,6 +648,17 @@ zero_pad(std::uint8_t* __restrict__ out, std::uint64_t nb, int U, std::uint64_t 
   const std::uint64_t x = nb * static_cast<std::uint64_t>(U / 8);
   for (std::uint64_t i = x + t; i < R; i += nthreads) {
     out[i] = 0;
+  }
+}
+
+// the exponent pad of a tplan: zpad bytes after exponent slot nb
+template<int U>
+__device__ __forceinline__ void
+zero_plan_pad(std::uint8_t* __restrict__ out, const tplan& p, int t, int nthreads)
+{
+  std::uint8_t* q = out + p.nb * static_cast<std::uint64_t>(U / 8);
+  for (std::uint32_t i = static_cast<std::uint32_t>(t); i < p.zpad; i += static_cast<std::uint32_t>(nthreads)) {
+    q[i] = 0;
   }
 }
 

@@ -16,13 +16,20 @@
 // stream and run asynchronously.  Every method reports a failure of the
 // library by throwing std::runtime_error with the library's message.
 //
-// Compiled by hipcc, the view also offers decompress_gpu_element(idx) for
-// use inside user kernels (include/frsz2.hpp:1096-1104).
+// compress_cpu_block / decompress_cpu_block keep their per-block meaning;
+// compress_gpu_blocks / decompress_gpu_blocks do the same for a run of
+// blocks in device memory.
+//
+// Compiled by hipcc, the view also offers, for use inside user kernels (the
+// view passed by value): decompress_gpu_element(idx)
+// (include/frsz2.hpp:1096-1104), and compress_gpu_block /
+// decompress_gpu_block, the per-block compress and decode by one thread.
 #ifndef FRSZ2_MI355X_HPP
 #define FRSZ2_MI355X_HPP
 
 #include "frsz2_capi.h"
 
+#include <algorithm>
 #include <climits>
 #include <cstddef>
 #include <cstdint>
@@ -156,6 +163,28 @@ struct frsz2_compressor
     return out;
   }
 
+  // compress_cpu_block, include/frsz2.hpp:1108-1179: block exp_block_id from
+  // its first exp_block_elements values (the rest of the block's codes are
+  // written as zero)
+  void compress_cpu_block(std::size_t exp_block_id, const fp_type* data,
+                          std::size_t exp_block_elements = max_exp_block_size)
+  {
+    detail::throw_if(frsz2_compress_blocks_host(detail::dtype_code<fp_type>(), bits_per_value, max_exp_block_size,
+                                                total_elements_, exp_block_id, data, exp_block_elements, data_,
+                                                stream_bytes()));
+  }
+
+  // decompress_cpu_block, include/frsz2.hpp:1217-1240
+  void decompress_cpu_block(std::size_t exp_block_id, fp_type* output) const
+  {
+    const std::size_t first = exp_block_id * max_exp_block_size;
+    const std::size_t count =
+      first < total_elements_ ? std::min<std::size_t>(max_exp_block_size, total_elements_ - first) : 0;
+    detail::throw_if(frsz2_decompress_blocks_host(detail::dtype_code<fp_type>(), bits_per_value, max_exp_block_size,
+                                                  total_elements_, exp_block_id, data_, stream_bytes(), count,
+                                                  output));
+  }
+
   // random-access decode of many indices (host memory)
   void decompress_cpu_elements(const std::uint64_t* idx, std::size_t m, fp_type* out) const
   {
@@ -187,6 +216,23 @@ struct frsz2_compressor
                                   stream_bytes(), total_elements_, d_idx, m, d_out, stream));
   }
 
+  // a run of blocks in device memory: values [first_block*B, +count)
+  void compress_gpu_blocks(std::size_t first_block, const fp_type* d_values, std::size_t count,
+                           frsz2_stream_t stream = nullptr)
+  {
+    detail::throw_if(frsz2_compress_blocks(detail::dtype_code<fp_type>(), bits_per_value, max_exp_block_size,
+                                           total_elements_, first_block, d_values, count, data_, stream_bytes(),
+                                           stream));
+  }
+
+  void decompress_gpu_blocks(std::size_t first_block, std::size_t count, fp_type* d_values,
+                             frsz2_stream_t stream = nullptr) const
+  {
+    detail::throw_if(frsz2_decompress_blocks(detail::dtype_code<fp_type>(), bits_per_value, max_exp_block_size,
+                                             total_elements_, first_block, data_, stream_bytes(), count, d_values,
+                                             stream));
+  }
+
 #if defined(__HIPCC__)
   // decompress_gpu_element, include/frsz2.hpp:1096-1104: decode one value
   // inside a kernel (the view is passed to the kernel by value)
@@ -194,6 +240,37 @@ struct frsz2_compressor
   {
     const frsz2::layout L = frsz2::layout::make(bits_per_value, max_exp_block_size, total_elements_);
     return frsz2::decode_element<fp_type, uint_compressed_size_bit>(data_, L, idx);
+  }
+
+  // compress_cpu_block inside a kernel, by one thread: block exp_block_id
+  // from its first exp_block_elements values (exponent slot and all payload
+  // bytes of the block; a kernel that compresses every block also clears the
+  // exponent pad with clear_exponent_pad)
+  __device__ void compress_gpu_block(std::size_t exp_block_id, const fp_type* values,
+                                     std::size_t exp_block_elements = max_exp_block_size)
+  {
+    const frsz2::layout L = frsz2::layout::make(bits_per_value, max_exp_block_size, total_elements_);
+    frsz2::encode_block<fp_type, uint_compressed_size_bit>(data_, L, exp_block_id, values, exp_block_elements);
+  }
+
+  // decompress_cpu_block inside a kernel, by one thread
+  __device__ void decompress_gpu_block(std::size_t exp_block_id, fp_type* output) const
+  {
+    const frsz2::layout L = frsz2::layout::make(bits_per_value, max_exp_block_size, total_elements_);
+    const std::size_t first = exp_block_id * max_exp_block_size;
+    const std::size_t count = total_elements_ - first < max_exp_block_size ? total_elements_ - first
+                                                                           : max_exp_block_size;
+    frsz2::decode_block<fp_type, uint_compressed_size_bit>(data_, L, exp_block_id, output, count);
+  }
+
+  // zero the exponent pad (byte i of it, for i = lane, lane + lanes, ...)
+  __device__ void clear_exponent_pad(unsigned lane, unsigned lanes)
+  {
+    const std::size_t nb = detail::ceildiv(total_elements_, max_exp_block_size);
+    const std::size_t end = compute_exponent_block_size_byte(total_elements_);
+    for (std::size_t i = nb * sizeof(exp_type) + lane; i < end; i += lanes) {
+      data_[i] = 0;
+    }
   }
 #endif
 

@@ -116,6 +116,45 @@ int frsz2_gather(int dtype,
                  void* d_out,
                  frsz2_stream_t stream);
 
+/* Compress part of an array into its stream: the `count` values at d_values
+ * are values [first_block*B, first_block*B + count) of an n-value array;
+ * they fill exponent blocks [first_block, first_block + ceil(count/B)) of the
+ * array's stream at d_stream (stream_bytes >= frsz2_compressed_size(n)),
+ * which is written at exactly those exponent slots and payload blocks, plus
+ * the zeroed exponent pad when the run ends at the stream's last block.
+ * The run's last block may be short of B values (a partial block inside the
+ * array, as exp_block_elements < B gives): its exponent is taken over the
+ * values given and its remaining codes are zero.  Runs over disjoint blocks
+ * may run concurrently and together give the bytes of frsz2_compress.
+ * Replaces frsz2_compressor::compress_cpu_block(exp_block_id, data,
+ * exp_block_elements) (include/frsz2.hpp:1108-1179), generalized to a run of
+ * blocks. */
+int frsz2_compress_blocks(int dtype,
+                          int bits,
+                          uint64_t block,
+                          uint64_t n,
+                          uint64_t first_block,
+                          const void* d_values,
+                          uint64_t count,
+                          void* d_stream,
+                          uint64_t stream_bytes,
+                          frsz2_stream_t stream);
+
+/* Decompress values [first_block*B, first_block*B + count) of the n-value
+ * array whose stream is at d_stream into d_values.
+ * Replaces frsz2_compressor::decompress_cpu_block(exp_block_id, output)
+ * (include/frsz2.hpp:1217-1240), generalized to a run of blocks. */
+int frsz2_decompress_blocks(int dtype,
+                            int bits,
+                            uint64_t block,
+                            uint64_t n,
+                            uint64_t first_block,
+                            const void* d_stream,
+                            uint64_t stream_bytes,
+                            uint64_t count,
+                            void* d_values,
+                            frsz2_stream_t stream);
+
 /* Kernel variant of the tiled plain kernels for (op, dtype, bits): op 0 =
  * compress, 1 = decompress; values_per_thread 2 (float64 with bits > 8
  * only; otherwise taken as 4), 4, 8 or 16: a tile is 256 x that many values
@@ -295,6 +334,13 @@ int frsz2_decompress_host(int dtype, int bits, uint64_t block, const void* h_in,
                           void* h_out);
 int frsz2_gather_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
                       const uint64_t* h_idx, uint64_t m, void* h_out);
+/* Host-memory variants of frsz2_compress_blocks / frsz2_decompress_blocks:
+ * h_stream is the whole host stream of the n-value array; only the run's
+ * exponent slots, payload blocks and (last run) exponent pad are copied. */
+int frsz2_compress_blocks_host(int dtype, int bits, uint64_t block, uint64_t n, uint64_t first_block,
+                               const void* h_values, uint64_t count, void* h_stream, uint64_t stream_bytes);
+int frsz2_decompress_blocks_host(int dtype, int bits, uint64_t block, uint64_t n, uint64_t first_block,
+                                 const void* h_stream, uint64_t stream_bytes, uint64_t count, void* h_values);
 
 #ifdef __cplusplus
 }

@@ -246,6 +246,69 @@ struct layout
   __host__ __device__ std::uint64_t total_bytes() const { return R + nb * CB; }
 };
 
+// Exponent slot k of a stream: the block exponent as a signed U-bit integer
+// (exp_type of include/frsz2.hpp:510, written at :1128).
+template<int U>
+__device__ __forceinline__ void
+store_block_exponent(std::uint8_t* __restrict__ stream, std::uint64_t k, int e)
+{
+  using wt = typename word<U>::type;
+  reinterpret_cast<wt*>(stream)[k] = static_cast<wt>(static_cast<std::int64_t>(e));
+}
+
+// In-kernel compress of exponent block k by one thread (compress_cpu_block,
+// include/frsz2.hpp:1108-1179): `values` holds the block's first `count`
+// inputs (count <= B; the stream's last block has n - k*B).  Writes exponent
+// slot k and all CB payload bytes of block k; codes past `count` are zero.
+// The exponent pad is not touched.  T = float/double, U = storage_bits(bits).
+template<typename T, int U>
+__device__ inline void
+encode_block(std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t k, const T* __restrict__ values,
+             std::uint64_t count)
+{
+  using uw = typename fmt<T>::uw;
+  using wt = typename word<U>::type;
+  std::uint32_t m = 0;
+  for (std::uint64_t i = 0; i < count; ++i) {
+    const std::uint32_t key = max_key<T>(__builtin_bit_cast(uw, values[i]));
+    m = m > key ? m : key;
+  }
+  const int E = block_exponent_from_key<T, U>(m);
+  store_block_exponent<U>(stream, k, E);
+  wt* words = reinterpret_cast<wt*>(stream + L.R + k * L.CB);
+  const std::uint64_t nwords = L.CB / (U / 8);
+  const std::uint64_t bits = static_cast<std::uint64_t>(L.bits);
+  for (std::uint64_t w = 0; w < nwords; ++w) {
+    // the codes that overlap bits [w*U, w*U + U) of the block
+    const std::uint64_t lo = w * U;
+    const std::uint64_t j0 = lo / bits;
+    std::uint64_t j1 = (lo + U + bits - 1) / bits;
+    j1 = j1 < count ? j1 : count;
+    std::uint64_t acc = 0;
+    for (std::uint64_t j = j0; j < j1; ++j) {
+      const std::uint64_t c = static_cast<std::uint64_t>(encode_value<T>(__builtin_bit_cast(uw, values[j]), E, L.bits));
+      const std::int64_t sh = static_cast<std::int64_t>(j * bits) - static_cast<std::int64_t>(lo);
+      acc |= sh >= 0 ? (c << sh) : (c >> (-sh));
+    }
+    words[w] = static_cast<wt>(acc);
+  }
+}
+
+// In-kernel decode of the first `count` values of exponent block k by one
+// thread (decompress_cpu_block, include/frsz2.hpp:1217-1240).
+template<typename T, int U>
+__device__ inline void
+decode_block(const std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t k, T* __restrict__ out,
+             std::uint64_t count)
+{
+  const int E = load_block_exponent<U>(stream, k);
+  const std::uint8_t* blk = stream + L.R + k * L.CB;
+  for (std::uint64_t j = 0; j < count; ++j) {
+    const std::uint64_t code = load_code<U>(blk, j, L.bits);
+    out[j] = __builtin_bit_cast(T, decode_value<T>(static_cast<typename fmt<T>::uw>(code), E, L.bits));
+  }
+}
+
 // Random-access decode of element idx of a stream (decompress_cpu_element,
 // include/frsz2.hpp:1199-1214), callable from any kernel.  T = float/double,
 // U = storage_bits(L.bits).

@@ -282,14 +282,13 @@ make_layout(int dtype, int bits, uint64_t B, uint64_t n, fo_layout* L)
   return 0;
 }
 
-/* compress_cpu_block(): include/frsz2.hpp:1108-1179.
+/* compress_cpu_block(exp_block_id = k, data = vals, exp_block_elements =
+ * len): include/frsz2.hpp:1108-1179.  vals points at the block's first value.
  * words: scratch of CB/(U/8) entries. */
 static void
-compress_block(const fo_layout* L, uint64_t k, const void* in, uint8_t* out, uint64_t* words)
+compress_block(const fo_layout* L, uint64_t k, const void* vals, uint64_t len, uint8_t* out, uint64_t* words)
 {
   const fo_fmt* f = L->f;
-  const uint64_t first = k * L->B;
-  const uint64_t len = (L->n - first) < L->B ? (L->n - first) : L->B;
   const int U = L->U;
   const int ub = U / 8;
 
@@ -300,7 +299,7 @@ compress_block(const fo_layout* L, uint64_t k, const void* in, uint8_t* out, uin
     float in_max = 0.0f;
     for (uint64_t i = 0; i < len; ++i) {
       float x;
-      memcpy(&x, (const uint8_t*)in + 4 * (first + i), 4);
+      memcpy(&x, (const uint8_t*)vals + 4 * i, 4);
       const float a = fabsf(x);
       if (in_max < a) {
         in_max = a;
@@ -313,7 +312,7 @@ compress_block(const fo_layout* L, uint64_t k, const void* in, uint8_t* out, uin
     double in_max = 0.0;
     for (uint64_t i = 0; i < len; ++i) {
       double x;
-      memcpy(&x, (const uint8_t*)in + 8 * (first + i), 8);
+      memcpy(&x, (const uint8_t*)vals + 8 * i, 8);
       const double a = fabs(x);
       if (in_max < a) {
         in_max = a;
@@ -330,7 +329,7 @@ compress_block(const fo_layout* L, uint64_t k, const void* in, uint8_t* out, uin
   memset(words, 0, nwords * sizeof(uint64_t));
   const uint64_t umask = wmask(U);
   for (uint64_t j = 0; j < len; ++j) {
-    const uint64_t fixed = to_fixed(load_bits(in, first + j, f->W), E16, f);
+    const uint64_t fixed = to_fixed(load_bits(vals, j, f->W), E16, f);
     /* uint_fp_to_compressed: top `bits` bits (include/frsz2.hpp:555-558) */
     const uint64_t code = (fixed >> (f->W - L->bits)) & umask;
     /* write_shift_value: include/frsz2.hpp:629-644 */
@@ -368,17 +367,60 @@ fetch_fixed(const fo_layout* L, const uint8_t* blk, uint64_t j)
   return (res << (L->f->W - L->bits)) & wmask(L->f->W);
 }
 
-/* decompress_cpu_block(): include/frsz2.hpp:1217-1240 */
+/* decompress_cpu_block(exp_block_id = k, output = vals): include/frsz2.hpp:1217-1240.
+ * vals receives the block's min(B, n - k*B) values. */
 static void
-decompress_block(const fo_layout* L, uint64_t k, const uint8_t* in, void* out)
+decompress_block(const fo_layout* L, uint64_t k, const uint8_t* in, void* vals)
 {
   const uint64_t first = k * L->B;
   const uint64_t len = (L->n - first) < L->B ? (L->n - first) : L->B;
   const int64_t E = load_exp(in, k, L->U);
   const uint8_t* blk = in + L->R + k * L->CB;
   for (uint64_t j = 0; j < len; ++j) {
-    store_bits(out, first + j, L->f->W, from_fixed(fetch_fixed(L, blk, j), E, L->f));
+    store_bits(vals, j, L->f->W, from_fixed(fetch_fixed(L, blk, j), E, L->f));
   }
+}
+
+/* compress_cpu_block(exp_block_id, data, exp_block_elements) of a view over
+ * n values (include/frsz2.hpp:1108-1179): the reference asserts
+ * exp_block_id * B + exp_block_elements <= n (:1115); here that is checked
+ * (returns 3) and exp_block_elements must also be 1..B. */
+int
+fo_compress_block(int dtype, int bits, uint64_t B, uint64_t n, uint64_t k, const void* vals, uint64_t len,
+                  uint8_t* out)
+{
+  fo_layout L;
+  const int rc = make_layout(dtype, bits, B, n, &L);
+  if (rc) {
+    return rc;
+  }
+  if (len == 0 || len > B || k >= L.nb || k * B + len > n) {
+    return 3;
+  }
+  uint64_t* words = (uint64_t*)malloc((L.CB / (uint64_t)(L.U / 8)) * sizeof(uint64_t));
+  if (!words) {
+    return 4;
+  }
+  compress_block(&L, k, vals, len, out, words);
+  free(words);
+  return 0;
+}
+
+/* decompress_cpu_block(exp_block_id, output): include/frsz2.hpp:1217-1240;
+ * writes min(B, n - k*B) values. */
+int
+fo_decompress_block(int dtype, int bits, uint64_t B, uint64_t n, uint64_t k, const uint8_t* in, void* vals)
+{
+  fo_layout L;
+  const int rc = make_layout(dtype, bits, B, n, &L);
+  if (rc) {
+    return rc;
+  }
+  if (k >= L.nb) {
+    return 3;
+  }
+  decompress_block(&L, k, in, vals);
+  return 0;
 }
 
 int
@@ -423,7 +465,9 @@ fo_compress_mt(int dtype, int bits, uint64_t B, const void* in, uint64_t n, uint
     } else {
 #pragma omp for schedule(static)
       for (int64_t k = 0; k < (int64_t)L.nb; ++k) {
-        compress_block(&L, (uint64_t)k, in, out, words);
+        const uint64_t first = (uint64_t)k * B;
+        const uint64_t len = (n - first) < B ? (n - first) : B;
+        compress_block(&L, (uint64_t)k, (const uint8_t*)in + first * (uint64_t)(L.f->W / 8), len, out, words);
       }
     }
     free(words);
@@ -450,7 +494,7 @@ fo_decompress_mt(int dtype, int bits, uint64_t B, const uint8_t* in, uint64_t n,
 #endif
 #pragma omp parallel for schedule(static) num_threads(nt)
   for (int64_t k = 0; k < (int64_t)L.nb; ++k) {
-    decompress_block(&L, (uint64_t)k, in, out);
+    decompress_block(&L, (uint64_t)k, in, (uint8_t*)out + (uint64_t)k * B * (uint64_t)(L.f->W / 8));
   }
   return 0;
 }

@@ -55,6 +55,14 @@ int fo_decompress(int dtype, int bits, uint64_t B, const uint8_t* in, uint64_t n
 int fo_gather(int dtype, int bits, uint64_t B, const uint8_t* in, uint64_t n, const uint64_t* idx,
               uint64_t m, void* out);
 
+/* one block (compress_cpu_block / decompress_cpu_block of a view over n
+ * values): vals holds the block's values; compress uses the first len
+ * (1..B, k*B + len <= n) and writes exponent k and the words its codes
+ * touch.  Returns 0, 1, 2, or 3 (block index / length out of range). */
+int fo_compress_block(int dtype, int bits, uint64_t B, uint64_t n, uint64_t k, const void* vals, uint64_t len,
+                      uint8_t* out);
+int fo_decompress_block(int dtype, int bits, uint64_t B, uint64_t n, uint64_t k, const uint8_t* in, void* vals);
+
 /* block-parallel variants over the same per-block routine (OpenMP, static
  * schedule); byte-identical to the serial ones. threads <= 0: all cores. */
 int fo_compress_mt(int dtype, int bits, uint64_t B, const void* in, uint64_t n, uint8_t* out, int threads);

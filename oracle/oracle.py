@@ -101,6 +101,27 @@ class _Codec:
             raise RuntimeError(f"{self.prefix}gather rc={rc}")
         return out
 
+    def compress_block(self, stream: np.ndarray, n: int, bits: int, B: int, k: int, vals: np.ndarray) -> None:
+        """compress_cpu_block(k, vals, vals.size) into `stream` (in place) of
+        a view over n values (include/frsz2.hpp:1108-1179)."""
+        vals = np.ascontiguousarray(vals)
+        rc = self._fn("compress_block")(_int(dtype_code(vals.dtype)), _int(bits), self._blk(B), _u64(n), _u64(k),
+                                        _vp(_ptr(vals)), _u64(vals.size), _vp(_ptr(stream)))
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}compress_block rc={rc}")
+
+    def decompress_block(self, stream: np.ndarray, n: int, dtype, bits: int, B: int, k: int) -> np.ndarray:
+        """decompress_cpu_block(k, out) of a view over n values
+        (include/frsz2.hpp:1217-1240): min(B, n - k*B) values."""
+        code = dtype_code(dtype)
+        out = np.empty(min(B, n - k * B), dtype=np_dtype(code))
+        stream = np.ascontiguousarray(stream, dtype=np.uint8)
+        rc = self._fn("decompress_block")(_int(code), _int(bits), self._blk(B), _u64(n), _u64(k),
+                                          _vp(_ptr(stream)), _vp(_ptr(out)))
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}decompress_block rc={rc}")
+        return out
+
 
 class Port(_Codec):
     """The C restatement (oracle/frsz2_oracle.c)."""
@@ -110,7 +131,7 @@ class Port(_Codec):
         L = self.lib
         L.fo_compressed_bytes.restype = _u64
         L.fo_compressed_bytes.argtypes = [_int, _u64, _u64]
-        for name in ("compress_mt", "decompress_mt"):
+        for name in ("compress_mt", "decompress_mt", "compress_block", "decompress_block"):
             getattr(L, "fo_" + name).restype = _int
         L.fo_gather.restype = _int
         L.fo_max_threads.restype = _int
@@ -164,7 +185,7 @@ class Reference(_Codec):
     def __init__(self, so_path: str = REF_SO):
         super().__init__(so_path, "ref_")
         L = self.lib
-        for name in ("compress", "decompress", "gather", "compressed_size"):
+        for name in ("compress", "decompress", "gather", "compressed_size", "compress_block", "decompress_block"):
             getattr(L, "ref_" + name).restype = _int
         for tag, ct in (("f32a", ctypes.c_float), ("f64a", ctypes.c_double)):
             getattr(L, "ref_exponent_" + tag).restype = ctypes.c_int32

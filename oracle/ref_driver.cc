@@ -106,6 +106,33 @@ REF_FN(ref_compress)(int bits, int B, const void* in, std::uint64_t n, std::uint
            : 2;
 }
 
+// compress_cpu_block(k, vals, len) on a view over n values
+int
+REF_FN(ref_compress_block)(int bits, int B, std::uint64_t n, std::uint64_t k, const void* vals, std::uint64_t len,
+                           std::uint8_t* out)
+{
+  return dispatch(bits, B, [&](auto t) {
+           using C = typename decltype(t)::type;
+           C c(out, n);
+           c.compress_cpu_block(k, static_cast<const T*>(vals), len);
+         })
+           ? 0
+           : 2;
+}
+
+// decompress_cpu_block(k, vals) on a view over n values
+int
+REF_FN(ref_decompress_block)(int bits, int B, std::uint64_t n, std::uint64_t k, const std::uint8_t* in, void* vals)
+{
+  return dispatch(bits, B, [&](auto t) {
+           using C = typename decltype(t)::type;
+           const C c(const_cast<std::uint8_t*>(in), n);
+           c.decompress_cpu_block(k, static_cast<T*>(vals));
+         })
+           ? 0
+           : 2;
+}
+
 int
 REF_FN(ref_decompress)(int bits, int B, const std::uint8_t* in, std::uint64_t n, void* out)
 {

@@ -8,7 +8,9 @@
   int ref_compressed_size_##tag(int, int, std::uint64_t, std::uint64_t*);                                  \
   int ref_compress_##tag(int, int, const void*, std::uint64_t, std::uint8_t*);                             \
   int ref_decompress_##tag(int, int, const std::uint8_t*, std::uint64_t, void*);                           \
-  int ref_gather_##tag(int, int, const std::uint8_t*, std::uint64_t, const std::uint64_t*, std::uint64_t, void*);
+  int ref_gather_##tag(int, int, const std::uint8_t*, std::uint64_t, const std::uint64_t*, std::uint64_t, void*); \
+  int ref_compress_block_##tag(int, int, std::uint64_t, std::uint64_t, const void*, std::uint64_t, std::uint8_t*); \
+  int ref_decompress_block_##tag(int, int, std::uint64_t, std::uint64_t, const std::uint8_t*, void*);
 
 extern "C" {
 REF_DECL(f32a)
@@ -75,6 +77,23 @@ ref_gather(int dtype,
 {
 #define ARGS bits, B, in, n, idx, m, out
   REF_CHAIN(ref_gather);
+#undef ARGS
+}
+
+int
+ref_compress_block(int dtype, int bits, int B, std::uint64_t n, std::uint64_t k, const void* vals, std::uint64_t len,
+                   std::uint8_t* out)
+{
+#define ARGS bits, B, n, k, vals, len, out
+  REF_CHAIN(ref_compress_block);
+#undef ARGS
+}
+
+int
+ref_decompress_block(int dtype, int bits, int B, std::uint64_t n, std::uint64_t k, const std::uint8_t* in, void* vals)
+{
+#define ARGS bits, B, n, k, in, vals
+  REF_CHAIN(ref_decompress_block);
 #undef ARGS
 }
 }

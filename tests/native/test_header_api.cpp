@@ -2,8 +2,9 @@
 // reference's GPU test drives its own header (test/test_frsz_gpu.cu:316-410):
 // the 9-value repeating pattern, double n = 2049 and float n = 2160, with the
 // same (bits, B) list, compress + decompress, lossless round trip.  Also the
-// device-pointer methods, the in-kernel decompress_gpu_element accessor and
-// the runtime dispatch with its error messages.
+// device-pointer methods, the per-block methods (host, device runs and in
+// kernel), the in-kernel decompress_gpu_element accessor and the runtime
+// dispatch with its error messages.
 //
 // Usage: test_header_api <outdir>
 //   writes <outdir>/<float|double>_b<bits>_B<B>.bin (the stream bytes) for
@@ -13,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -44,6 +46,32 @@ element_kernel(C view, std::size_t n, typename C::fp_type* out)
   }
 }
 
+// in-kernel per-block compress / decode: one thread per exponent block
+template<class C>
+__global__ void
+block_compress_kernel(C view, const typename C::fp_type* in, std::size_t n)
+{
+  constexpr std::size_t B = C::max_exp_block_size;
+  const std::size_t k = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0) {
+    view.clear_exponent_pad(threadIdx.x, blockDim.x);
+  }
+  if (k * B < n) {
+    view.compress_gpu_block(k, in + k * B, n - k * B < B ? n - k * B : B);
+  }
+}
+
+template<class C>
+__global__ void
+block_decode_kernel(C view, typename C::fp_type* out, std::size_t n)
+{
+  constexpr std::size_t B = C::max_exp_block_size;
+  const std::size_t k = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k * B < n) {
+    view.decompress_gpu_block(k, out + k * B);
+  }
+}
+
 template<int bits, int B, class T>
 void
 run_case(const std::vector<T>& in, const std::string& outdir)
@@ -64,6 +92,22 @@ run_case(const std::vector<T>& in, const std::string& outdir)
   for (std::size_t i : { std::size_t{ 0 }, n / 3, n - 1 }) {
     const T e = host_view.decompress_cpu_element(i);
     CHECK(std::memcmp(&e, &out[i], sizeof(T)) == 0, "decompress_cpu_element(%zu)", i);
+  }
+  // per-block methods, every block into a buffer of other contents
+  {
+    const std::size_t nb = (n + B - 1) / B;
+    std::vector<std::uint8_t> s2(size, 0xA5);
+    C v2(s2.data(), n);
+    for (std::size_t k = 0; k < nb; ++k) {
+      v2.compress_cpu_block(k, in.data() + k * B, std::min<std::size_t>(B, n - k * B));
+    }
+    CHECK(s2 == stream, "%s b%d B%d: compress_cpu_block stream differs", tname, bits, B);
+    std::vector<T> o2(n, T(-1));
+    for (std::size_t k = 0; k < nb; ++k) {
+      v2.decompress_cpu_block(k, o2.data() + k * B);
+    }
+    CHECK(std::memcmp(o2.data(), out.data(), n * sizeof(T)) == 0, "%s b%d B%d: decompress_cpu_block differs", tname,
+          bits, B);
   }
   const std::string path = outdir + "/" + tname + "_b" + std::to_string(bits) + "_B" + std::to_string(B) + ".bin";
   if (FILE* f = std::fopen(path.c_str(), "wb")) {
@@ -94,6 +138,32 @@ run_case(const std::vector<T>& in, const std::string& outdir)
   HIP_OK(hipDeviceSynchronize());
   HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
   CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "decompress_gpu_element differs");
+  // in-kernel per-block compress and decode
+  {
+    const std::size_t nb = (n + B - 1) / B;
+    HIP_OK(hipMemset(d_stream, 0x3C, size));
+    block_compress_kernel<C><<<dim3((nb + 63) / 64), dim3(64)>>>(dev_view, d_in, n);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+    CHECK(dstream == stream, "%s b%d B%d: compress_gpu_block stream differs", tname, bits, B);
+    HIP_OK(hipMemset(d_out, 0, n * sizeof(T)));
+    block_decode_kernel<C><<<dim3((nb + 63) / 64), dim3(64)>>>(dev_view, d_out, n);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+    CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "decompress_gpu_block differs");
+    // two runs of blocks through the device-memory run methods
+    const std::size_t half = nb / 2;
+    HIP_OK(hipMemset(d_stream, 0x77, size));
+    dev_view.compress_gpu_blocks(half, d_in + half * B, n - half * B);
+    dev_view.compress_gpu_blocks(0, d_in, half * B);
+    HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+    CHECK(dstream == stream, "%s b%d B%d: compress_gpu_blocks stream differs", tname, bits, B);
+    HIP_OK(hipMemset(d_out, 0, n * sizeof(T)));
+    dev_view.decompress_gpu_blocks(0, half * B, d_out);
+    dev_view.decompress_gpu_blocks(half, n - half * B, d_out + half * B);
+    HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+    CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "decompress_gpu_blocks differs");
+  }
   HIP_OK(hipFree(d_in));
   HIP_OK(hipFree(d_out));
   HIP_OK(hipFree(d_stream));

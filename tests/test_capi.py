@@ -93,3 +93,19 @@ def test_python_plugin_surface_defaults():
     c.set_options({"frsz2:bits": 16})
     assert c.clone().get_options()["frsz2:bits"] == 16
     assert c.get_configuration()["pressio:thread_safe"] == "multiple"
+
+
+def test_block_run_errors():
+    fake = ctypes.c_void_p(1 << 20)
+    L = capi.lib
+    # 1000 values, B 32 -> 32 blocks; a run from block 31 holds at most 8 values
+    assert L.frsz2_compress_blocks(0, 16, 32, 1000, 31, fake, 9, fake, 1 << 20, None) == capi.EARG
+    assert "exceed" in capi.last_error()
+    assert L.frsz2_compress_blocks(0, 16, 32, 1000, 33, fake, 0, fake, 1 << 20, None) == capi.EARG
+    assert L.frsz2_decompress_blocks(0, 16, 32, 1000, 0, fake, 1 << 20, 1001, fake, None) == capi.EARG
+    assert L.frsz2_compress_blocks(0, 16, 32, 1000, 0, fake, 10, fake, 100, None) == capi.ECAPACITY
+    assert L.frsz2_compress_blocks(0, 3, 32, 1000, 0, fake, 10, fake, 1 << 20, None) == 2
+    assert L.frsz2_compress_blocks_host(0, 16, 32, 1000, 31, fake, 9, fake, 1 << 20) == capi.EARG
+    assert L.frsz2_decompress_blocks_host(2, 16, 32, 1000, 0, fake, 1 << 20, 10, fake) == capi.EDTYPE
+    # a run with no values is a no-op unless it ends the stream (then it writes the pad)
+    assert L.frsz2_compress_blocks_host(0, 16, 32, 1000, 3, None, 0, fake, 1 << 20) == 0

@@ -198,3 +198,30 @@ def test_value_helpers_match_compiled_reference(port, reference):
         a = np.float32(R.ref_fixed_to_floating_f32a(fx, E)).view(np.uint32)
         b = np.float32(port.lib.fo_fixed_to_floating_f32(fx, E)).view(np.uint32)
         assert a == b
+
+
+def test_block_entry_points_match_compiled_reference(port, reference):
+    """compress_cpu_block(k, data, exp_block_elements) with any length up to
+    the block (a partial block inside the array included) and
+    decompress_cpu_block, into buffers of arbitrary prior contents."""
+    rng = np.random.default_rng(11)
+    for dtype in (np.float32, np.float64):
+        W = np.dtype(dtype).itemsize * 8
+        for bits in (4, 7, 8, 9, 12, 16, 21, 31, 32) + ((33, 40, 52, 64) if W == 64 else ()):
+            for B in (1, 3, 5, 8, 9, 32, 128):
+                n = int(rng.integers(1, 700))
+                x = (rng.standard_normal(n) * np.exp2(rng.integers(-20, 20, n))).astype(dtype)
+                x[rng.random(n) < 0.1] = 0
+                nb = -(-n // B)
+                for _ in range(2):
+                    k = int(rng.integers(0, nb))
+                    ln = int(rng.integers(1, min(B, n - k * B) + 1))
+                    s1 = np.full(compressed_bytes(bits, B, n), int(rng.integers(0, 256)), np.uint8)
+                    s2 = s1.copy()
+                    port.compress_block(s1, n, bits, B, k, x[k * B: k * B + ln])
+                    reference.compress_block(s2, n, bits, B, k, x[k * B: k * B + ln])
+                    assert np.array_equal(s1, s2), (dtype, bits, B, n, k, ln)
+                    a = port.decompress_block(s1, n, dtype, bits, B, k)
+                    b = reference.decompress_block(s2, n, dtype, bits, B, k)
+                    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
