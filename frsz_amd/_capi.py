@@ -15,12 +15,12 @@ LIB_PATH = os.path.join(HERE, "lib", "libfrsz2_hip.so")
 FLOAT32 = 0
 FLOAT64 = 1
 
-OK, EDTYPE, EPARAM, ECAPACITY, EARG, EHIP = range(6)
+OK, EDTYPE, EPARAM, ECAPACITY, EARG, EHIP, EFORMAT = range(7)
 
 
 class Frsz2Error(RuntimeError):
     """A non-zero status of the C ABI; ``code`` follows the reference plugin
-    (1 unsupported dtype, 2 unsupported bits / block size) plus 3..5."""
+    (1 unsupported dtype, 2 unsupported bits / block size) plus 3..6."""
 
     def __init__(self, code: int, message: str):
         super().__init__(f"frsz2 error {code}: {message}")
@@ -81,6 +81,26 @@ lib.frsz2_compress_blocks_host.restype = _int
 lib.frsz2_compress_blocks_host.argtypes = [_int, _int, _u64, _u64, _u64, _vp, _u64, _vp, _u64]
 lib.frsz2_decompress_blocks_host.restype = _int
 lib.frsz2_decompress_blocks_host.argtypes = [_int, _int, _u64, _u64, _u64, _vp, _u64, _u64, _vp]
+
+
+class ContainerInfo(ctypes.Structure):
+    _fields_ = [("dtype", _int), ("bits", _int), ("block", _u64), ("n", _u64), ("stream_bytes", _u64),
+                ("header_bytes", _u64)]
+
+
+CONTAINER_HEADER_BYTES = 64
+lib.frsz2_container_size.restype = _int
+lib.frsz2_container_size.argtypes = [_int, _int, _u64, _u64, _u64p]
+lib.frsz2_container_parse.restype = _int
+lib.frsz2_container_parse.argtypes = [_vp, _u64, ctypes.POINTER(ContainerInfo)]
+lib.frsz2_compress_container.restype = _int
+lib.frsz2_compress_container.argtypes = [_int, _int, _u64, _vp, _u64, _vp, _u64, _vp]
+lib.frsz2_decompress_container.restype = _int
+lib.frsz2_decompress_container.argtypes = [ctypes.POINTER(ContainerInfo), _vp, _u64, _vp, _u64, _vp]
+lib.frsz2_compress_container_host.restype = _int
+lib.frsz2_compress_container_host.argtypes = [_int, _int, _u64, _vp, _u64, _vp, _u64]
+lib.frsz2_decompress_container_host.restype = _int
+lib.frsz2_decompress_container_host.argtypes = [_vp, _u64, _vp, _u64]
 
 
 def check(rc: int) -> None:

@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(LIBDIR, "libfrsz2_hip.so")
 ARCH = "gfx950"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["frsz2_plain.hip", "frsz2_host.hip", "frsz2_pipeline.hip", "frsz2_batch.hip", "frsz2_serve.hip"]
+HIP_SOURCES = ["frsz2_plain.hip", "frsz2_host.hip", "frsz2_pipeline.hip", "frsz2_container.hip", "frsz2_batch.hip", "frsz2_serve.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -20,8 +20,8 @@
  * (src/frsz2.cc:164, :181), 2 unsupported bits / block size with the
  * reference's message text "Unsupported number of bits: N" /
  * "Unsupported exponent block size: N" (include/frsz2.hpp:1300, :1326).
- * Codes 3..5 are new: capacity too small, bad pointer or alignment, HIP
- * runtime error.  frsz2_last_error() returns the message of the calling
+ * Codes 3..6 are new: capacity too small, bad pointer or alignment, HIP
+ * runtime error, malformed container header.  frsz2_last_error() returns the message of the calling
  * thread's last failure.
  */
 #ifndef FRSZ2_CAPI_H
@@ -49,7 +49,8 @@ enum
   FRSZ2_EPARAM = 2,
   FRSZ2_ECAPACITY = 3,
   FRSZ2_EARG = 4,
-  FRSZ2_EHIP = 5
+  FRSZ2_EHIP = 5,
+  FRSZ2_EFORMAT = 6
 };
 
 #define FRSZ2_MAX_BLOCK (1u << 24)
@@ -324,10 +325,57 @@ int frsz2_server_replay(frsz2_server* server, const frsz2_request* reqs, size_t 
 int frsz2_server_get_stats(frsz2_server* server, frsz2_server_stats* stats);
 int frsz2_server_destroy(frsz2_server* server);
 
-/* Host-memory variants: stage through device memory, run the HIP kernels and
- * copy the result back; synchronous.  They keep the host-in / host-out
+/* ---- self-describing container ----
+ * The reference stream carries no header: n, dtype, bits and B travel out of
+ * band (src/frsz2.cc:152-186).  A container is a 64-byte header (magic
+ * "FRSZ2CT", version 1, dtype, bits, B, n, stream bytes; little-endian)
+ * followed by the unchanged stream, so container[64:] is exactly what
+ * frsz2_compress writes.  No counterpart in the reference. */
+#define FRSZ2_CONTAINER_HEADER_BYTES 64u
+
+typedef struct frsz2_container_info
+{
+  int dtype;
+  int bits;
+  uint64_t block;
+  uint64_t n;
+  uint64_t stream_bytes;
+  uint64_t header_bytes; /* offset of the stream in the container (64) */
+} frsz2_container_info;
+
+/* Header + stream bytes for n values.  Host-only. */
+int frsz2_container_size(int dtype, int bits, uint64_t block, uint64_t n, uint64_t* bytes);
+/* Parse and check a header in host memory (at least its first 64 bytes;
+ * `bytes` = how many are there): magic, version, supported (dtype, bits, B)
+ * and a stream size that matches them; 6 if malformed.  Host-only. */
+int frsz2_container_parse(const void* h_container, uint64_t bytes, frsz2_container_info* info);
+/* Write header + stream of n values at d_in into d_out (8-byte aligned;
+ * 64-byte aligned keeps the stream on the tiled kernels). */
+int frsz2_compress_container(int dtype,
+                             int bits,
+                             uint64_t block,
+                             const void* d_in,
+                             uint64_t n,
+                             void* d_out,
+                             uint64_t out_capacity,
+                             frsz2_stream_t stream);
+/* Decompress the container at d_in, described by `info` (frsz2_container_parse
+ * of a host copy of its header), into d_out (room for out_values >= n). */
+int frsz2_decompress_container(const frsz2_container_info* info,
+                               const void* d_in,
+                               uint64_t in_bytes,
+                               void* d_out,
+                               uint64_t out_values,
+                               frsz2_stream_t stream);
+
+/* Host-memory variants; synchronous.  They keep the host-in / host-out
  * meaning of the reference's CPU entry points and of the LibPressio plugin
- * (src/frsz2.cc:152-186), which dereference host buffers. */
+ * (src/frsz2.cc:152-186), which dereference host buffers.  Compress and
+ * decompress run as a pipeline over chunks of about 32 MiB of values (whole
+ * exponent blocks): CPU copies into pinned staging, H2D, kernel and D2H of
+ * different chunks overlap on three streams; staging stays allocated for
+ * reuse until the process ends.  Gather copies the stream and the indices
+ * to the device, runs the gather kernel and copies the values back. */
 int frsz2_compress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t n, void* h_out,
                         uint64_t out_capacity);
 int frsz2_decompress_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t in_bytes, uint64_t n,
@@ -337,6 +385,11 @@ int frsz2_gather_host(int dtype, int bits, uint64_t block, const void* h_in, uin
 /* Host-memory variants of frsz2_compress_blocks / frsz2_decompress_blocks:
  * h_stream is the whole host stream of the n-value array; only the run's
  * exponent slots, payload blocks and (last run) exponent pad are copied. */
+/* Container in host memory (e.g. a file image): compress writes header +
+ * stream; decompress parses its own header.  Synchronous. */
+int frsz2_compress_container_host(int dtype, int bits, uint64_t block, const void* h_in, uint64_t n, void* h_out,
+                                  uint64_t out_capacity);
+int frsz2_decompress_container_host(const void* h_in, uint64_t in_bytes, void* h_out, uint64_t out_values);
 int frsz2_compress_blocks_host(int dtype, int bits, uint64_t block, uint64_t n, uint64_t first_block,
                                const void* h_values, uint64_t count, void* h_stream, uint64_t stream_bytes);
 int frsz2_decompress_blocks_host(int dtype, int bits, uint64_t block, uint64_t n, uint64_t first_block,

@@ -109,3 +109,48 @@ def test_block_run_errors():
     assert L.frsz2_decompress_blocks_host(2, 16, 32, 1000, 0, fake, 1 << 20, 10, fake) == capi.EDTYPE
     # a run with no values is a no-op unless it ends the stream (then it writes the pad)
     assert L.frsz2_compress_blocks_host(0, 16, 32, 1000, 3, None, 0, fake, 1 << 20) == 0
+
+
+def _header(dtype=0, bits=16, block=32, n=1000, stream=None, magic=b"FRSZ2CT\0", version=1, hdr=64):
+    import struct
+
+    from oracle.oracle import compressed_bytes
+
+    if stream is None:
+        stream = compressed_bytes(bits, block, n)
+    return bytearray(struct.pack("<8sIIIIQQQ16x", magic, version, dtype, bits, hdr, block, n, stream))
+
+
+def _parse(buf):
+    info = capi.ContainerInfo()
+    h = np.frombuffer(bytes(buf), dtype=np.uint8)
+    rc = capi.lib.frsz2_container_parse(h.ctypes.data, h.size, ctypes.byref(info))
+    return rc, info
+
+
+def test_container_header_parse():
+    rc, info = _parse(_header(1, 21, 5, 2049))
+    assert rc == 0
+    assert (info.dtype, info.bits, info.block, info.n, info.header_bytes) == (1, 21, 5, 2049, 64)
+    from oracle.oracle import compressed_bytes
+
+    assert info.stream_bytes == compressed_bytes(21, 5, 2049)
+    size = ctypes.c_uint64()
+    assert capi.lib.frsz2_container_size(1, 21, 5, 2049, ctypes.byref(size)) == 0
+    assert size.value == 64 + info.stream_bytes
+
+
+@pytest.mark.parametrize("buf,rc", [
+    (_header(magic=b"FRSZ2XX\0"), capi.EFORMAT),
+    (_header(version=2), capi.EFORMAT),
+    (_header(hdr=128), capi.EFORMAT),
+    (_header(stream=12345), capi.EFORMAT),
+    (_header()[:40], capi.EFORMAT),
+    (_header(bits=33), capi.EPARAM),
+    (_header(dtype=3), capi.EDTYPE),
+    (_header(block=0, stream=256), capi.EPARAM),
+])
+def test_container_header_rejects(buf, rc):
+    got, _ = _parse(buf)
+    assert got == rc
+    assert capi.last_error()
