@@ -119,18 +119,19 @@ def version() -> str:
 TUNE_NONTEMPORAL = 1
 TUNE_STAGED_STORES = 2
 TUNE_LINEAR_ORDER = 4
+TUNE_SMALL_WORKGROUPS = 8
 
 
 def tune_set(op: int, dtype: int, bits: int, values_per_thread: int, nontemporal: bool = True,
-             staged_stores: bool = False, linear_order: bool = False) -> None:
+             staged_stores: bool = False, linear_order: bool = False, small_workgroups: bool = False) -> None:
     flags = ((TUNE_NONTEMPORAL if nontemporal else 0) | (TUNE_STAGED_STORES if staged_stores else 0)
-             | (TUNE_LINEAR_ORDER if linear_order else 0))
+             | (TUNE_LINEAR_ORDER if linear_order else 0) | (TUNE_SMALL_WORKGROUPS if small_workgroups else 0))
     check(lib.frsz2_tune_set(op, dtype, bits, values_per_thread, flags))
 
 
 def tune_get(op: int, dtype: int, bits: int):
-    """(values_per_thread, nontemporal, staged_stores, linear_order) of (op, dtype, bits)."""
+    """(values_per_thread, nontemporal, staged_stores, linear_order, small_workgroups) of (op, dtype, bits)."""
     v, fl = ctypes.c_int(), ctypes.c_int()
     check(lib.frsz2_tune_get(op, dtype, bits, ctypes.byref(v), ctypes.byref(fl)))
     return (v.value, bool(fl.value & TUNE_NONTEMPORAL), bool(fl.value & TUNE_STAGED_STORES),
-            bool(fl.value & TUNE_LINEAR_ORDER))
+            bool(fl.value & TUNE_LINEAR_ORDER), bool(fl.value & TUNE_SMALL_WORKGROUPS))

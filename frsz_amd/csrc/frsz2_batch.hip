@@ -326,7 +326,7 @@ k_reset_counter(unsigned long long* counter)
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP>
 __global__ __launch_bounds__(WG) void k_grouped(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base)
 {
-  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT>() : 1];
   __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT>() : 1];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
   const std::uint64_t g = tile_base + blockIdx.x;

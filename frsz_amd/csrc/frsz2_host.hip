@@ -59,15 +59,15 @@ get_variant(int op, int dtype, int bits)
 {
   init_variants();
   const int v = g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].load(std::memory_order_relaxed);
-  return variant{ v / 8, v % 2, (v / 2) % 2, (v / 4) % 2 };
+  return variant{ venc_vpt(v), venc_nt(v), venc_stage(v), venc_linear(v), venc_small(v) };
 }
 
 int
-set_variant(int op, int dtype, int bits, int vpt, int nt, int stage, int linear)
+set_variant(int op, int dtype, int bits, int vpt, int nt, int stage, int linear, int small)
 {
   init_variants();
   g_variant[op][dtype][u_index(bits)][bits < storage_bits(bits) ? 1 : 0].store(
-    venc(vpt, stage ? 1 : 0, nt ? 1 : 0, linear ? 1 : 0), std::memory_order_relaxed);
+    venc(vpt, stage ? 1 : 0, nt ? 1 : 0, linear ? 1 : 0, small ? 1 : 0), std::memory_order_relaxed);
   return 0;
 }
 
@@ -138,8 +138,11 @@ frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags)
       (values_per_thread != 2 && values_per_thread != 4 && values_per_thread != 8 && values_per_thread != 16)) {
     return set_error(FRSZ2_EARG, "op must be 0/1 and values_per_thread 2, 4, 8 or 16");
   }
-  return set_variant(op, dtype, bits, values_per_thread, flags & FRSZ2_TUNE_NONTEMPORAL,
-                     op == OP_DECOMPRESS ? (flags & FRSZ2_TUNE_STAGED_STORES) : 0, flags & FRSZ2_TUNE_LINEAR_ORDER);
+  const int small = (flags & FRSZ2_TUNE_SMALL_WORKGROUPS) != 0;
+  // 64-thread workgroups are built with the non-temporal hint only
+  return set_variant(op, dtype, bits, values_per_thread, small || (flags & FRSZ2_TUNE_NONTEMPORAL),
+                     op == OP_DECOMPRESS ? (flags & FRSZ2_TUNE_STAGED_STORES) : 0, flags & FRSZ2_TUNE_LINEAR_ORDER,
+                     small);
 }
 
 int
@@ -157,7 +160,7 @@ frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags)
   const variant v = get_variant(op, dtype, bits);
   *values_per_thread = v.vpt;
   *flags = (v.nt ? FRSZ2_TUNE_NONTEMPORAL : 0) | (v.stage ? FRSZ2_TUNE_STAGED_STORES : 0) |
-           (v.linear ? FRSZ2_TUNE_LINEAR_ORDER : 0);
+           (v.linear ? FRSZ2_TUNE_LINEAR_ORDER : 0) | (v.small ? FRSZ2_TUNE_SMALL_WORKGROUPS : 0);
   return FRSZ2_OK;
 }
 

@@ -30,9 +30,10 @@ struct variant
   int nt;
   int stage;
   int linear; // tiles in workgroup order instead of one contiguous range per XCD
+  int small;  // 64-thread workgroups (one wave) instead of 256
 };
 variant get_variant(int op, int dtype, int bits);
-int set_variant(int op, int dtype, int bits, int vpt, int nt, int stage, int linear);
+int set_variant(int op, int dtype, int bits, int vpt, int nt, int stage, int linear, int small);
 int hip_status(hipError_t e, const char* what);
 
 // A run of exponent blocks [first, first + nb) of the stream of an n-value

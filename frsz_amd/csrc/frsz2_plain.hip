@@ -29,33 +29,36 @@ using tile::WG;
 
 // ---------------------------------------------------------------- kernels
 
-template<typename T, int U, int VPT, bool PACKED, bool NT>
-__global__ __launch_bounds__(WG) void k_compress_tile(const typename fmt<T>::uw* __restrict__ in,
-                                                      std::uint8_t* __restrict__ out,
-                                                      tplan p,
-                                                      std::uint32_t ntiles,
-                                                      int linear)
+// WGS: threads per workgroup, 256 or 64 (one wave; the tile is then a
+// quarter as large and a workgroup's resources free as soon as its one wave
+// ends)
+template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS>
+__global__ __launch_bounds__(WGS) void k_compress_tile(const typename fmt<T>::uw* __restrict__ in,
+                                                       std::uint8_t* __restrict__ out,
+                                                       tplan p,
+                                                       std::uint32_t ntiles,
+                                                       int linear)
 {
-  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
-  __shared__ std::uint32_t wred[WG / tile::WAVE];
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
+  __shared__ std::uint32_t wred[WGS / tile::WAVE];
   const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
-  tile::compress_tile<T, U, VPT, PACKED, NT>(in, out, p, t, lds, wred);
+  tile::compress_tile<T, U, VPT, PACKED, NT, WGS>(in, out, p, t, lds, wred);
   if (t == 0) {
-    tile::zero_plan_pad<U>(out, p, static_cast<int>(threadIdx.x), WG);
+    tile::zero_plan_pad<U>(out, p, static_cast<int>(threadIdx.x), WGS);
   }
 }
 
-template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE>
-__global__ __launch_bounds__(WG) void k_decompress_tile(const std::uint8_t* __restrict__ in,
-                                                        typename fmt<T>::uw* __restrict__ out,
-                                                        tplan p,
-                                                        std::uint32_t ntiles,
-                                                        int linear)
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int WGS>
+__global__ __launch_bounds__(WGS) void k_decompress_tile(const std::uint8_t* __restrict__ in,
+                                                         typename fmt<T>::uw* __restrict__ out,
+                                                         tplan p,
+                                                         std::uint32_t ntiles,
+                                                         int linear)
 {
-  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords<VPT>() : 1];
-  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT>() : 1];
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
+  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
   const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
-  tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE>(in, out, p, t, lds, stage);
+  tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE, WGS>(in, out, p, t, lds, stage);
 }
 
 // One thread per exponent block, any B (encode_block: each payload word is
@@ -130,12 +133,12 @@ grid_for(std::uint64_t work)
   return static_cast<unsigned>((work + WG - 1) / WG);
 }
 
-template<typename T, int U, int VPT, bool NT>
+template<typename T, int U, int VPT, bool NT, int WGS>
 hipError_t
 launch_compress_tiled(const void* in, void* out, const layout& L, std::uint32_t zpad, int lb, int linear,
                       hipStream_t s)
 {
-  constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
+  constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WGS) * VPT;
   tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, zpad };
   std::uint64_t nt = (L.n + TILE - 1) / TILE;
   if (nt == 0) {
@@ -144,20 +147,20 @@ launch_compress_tiled(const void* in, void* out, const layout& L, std::uint32_t 
   const auto* x = static_cast<const typename fmt<T>::uw*>(in);
   auto* o = static_cast<std::uint8_t*>(out);
   if (L.bits == U) {
-    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false, NT, WGS>), dim3(nt), dim3(WGS), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt), linear);
   } else {
-    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true, NT>), dim3(nt), dim3(WG), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true, NT, WGS>), dim3(nt), dim3(WGS), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt), linear);
   }
   return hipGetLastError();
 }
 
-template<typename T, int U, int VPT, bool NT, bool STAGE>
+template<typename T, int U, int VPT, bool NT, bool STAGE, int WGS>
 hipError_t
 launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, int linear, hipStream_t s)
 {
-  constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WG) * VPT;
+  constexpr std::uint64_t TILE = static_cast<std::uint64_t>(WGS) * VPT;
   tplan p{ L.n, L.nb, L.R, static_cast<std::uint32_t>(L.CB), L.bits, lb, 0u };
   const std::uint64_t nt = (L.n + TILE - 1) / TILE;
   if (nt == 0) {
@@ -166,10 +169,10 @@ launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, int 
   const auto* x = static_cast<const std::uint8_t*>(in);
   auto* o = static_cast<typename fmt<T>::uw*>(out);
   if (L.bits == U) {
-    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false, NT, STAGE>), dim3(nt), dim3(WG), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, false, NT, STAGE, WGS>), dim3(nt), dim3(WGS), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt), linear);
   } else {
-    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true, NT, STAGE>), dim3(nt), dim3(WG), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_decompress_tile<T, U, VPT, true, NT, STAGE, WGS>), dim3(nt), dim3(WGS), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt), linear);
   }
   return hipGetLastError();
@@ -210,48 +213,64 @@ launch_gather(const void* in, const layout& L, const std::uint64_t* idx, std::ui
 }
 
 // Values per thread actually launched: the tuned one, raised to the smallest
-// supported one whose tile (256 x VPT values) holds a whole exponent block.
+// supported one whose tile (WGS x VPT values) holds a whole exponent block.
 // 2 values per thread exists for float64 with U >= 16 only (one 16-byte load
 // per lane).  0 = no tile holds the block (the one-thread-per-block kernels).
 template<typename T, int U>
 int
-effective_vpt(int vpt, std::uint64_t B)
+effective_vpt(int vpt, std::uint64_t B, int wgs)
 {
   if (vpt == 2 && !(sizeof(T) == 8 && U >= 16)) {
     vpt = 4;
   }
-  while (vpt <= 16 && static_cast<std::uint64_t>(WG) * vpt < B) {
+  while (vpt <= 16 && static_cast<std::uint64_t>(wgs) * vpt < B) {
     vpt *= 2;
   }
   return vpt <= 16 ? vpt : 0;
 }
 
 // Tiled launch with the variant of the tuning table (values per thread,
-// non-temporal hint, tile order, staged decompress stores); B beyond the
-// largest tile, or unaligned pointers, take the one-thread-per-block kernels.
+// non-temporal hint, tile order, staged decompress stores, workgroup size);
+// B beyond the largest tile, or unaligned pointers, take the
+// one-thread-per-block kernels.  Workgroups of 64 threads are built with the
+// non-temporal hint only.
 template<typename T, int U>
 hipError_t
 run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
-  const int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B) : 0;
+  const int wgs = v.small ? 64 : WG;
+  int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B, wgs) : 0;
   if (vpt > 0) {
-#define FRSZ2_CL(V, N) return launch_compress_tiled<T, U, V, N>(in, out, L, zpad, lb, v.linear, s)
+#define FRSZ2_CL(V, N, W) return launch_compress_tiled<T, U, V, N, W>(in, out, L, zpad, lb, v.linear, s)
+    if (v.small) {
+      if constexpr (sizeof(T) == 8 && U >= 16) {
+        if (vpt == 2) {
+          FRSZ2_CL(2, true, 64);
+        }
+      }
+      vpt = vpt < 4 ? 4 : vpt;
+      switch (vpt) {
+        case 4: FRSZ2_CL(4, true, 64);
+        case 8: FRSZ2_CL(8, true, 64);
+        default: FRSZ2_CL(16, true, 64);
+      }
+    }
     if constexpr (sizeof(T) == 8 && U >= 16) {
       if (vpt == 2) {
         if (v.nt) {
-          FRSZ2_CL(2, true);
+          FRSZ2_CL(2, true, WG);
         }
-        FRSZ2_CL(2, false);
+        FRSZ2_CL(2, false, WG);
       }
     }
     switch (vpt * 2 + v.nt) {
-      case 8: FRSZ2_CL(4, false);
-      case 9: FRSZ2_CL(4, true);
-      case 16: FRSZ2_CL(8, false);
-      case 17: FRSZ2_CL(8, true);
-      case 32: FRSZ2_CL(16, false);
-      default: FRSZ2_CL(16, true);
+      case 8: FRSZ2_CL(4, false, WG);
+      case 9: FRSZ2_CL(4, true, WG);
+      case 16: FRSZ2_CL(8, false, WG);
+      case 17: FRSZ2_CL(8, true, WG);
+      case 32: FRSZ2_CL(16, false, WG);
+      default: FRSZ2_CL(16, true, WG);
     }
 #undef FRSZ2_CL
   }
@@ -263,32 +282,52 @@ hipError_t
 run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
-  const int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B) : 0;
+  const int wgs = v.small ? 64 : WG;
+  int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B, wgs) : 0;
   if (vpt > 0) {
-#define FRSZ2_DL(V, N, S) return launch_decompress_tiled<T, U, V, N, S>(in, out, L, lb, v.linear, s)
+#define FRSZ2_DL(V, N, S, W) return launch_decompress_tiled<T, U, V, N, S, W>(in, out, L, lb, v.linear, s)
+    if (v.small) {
+      if constexpr (sizeof(T) == 8 && U >= 16) {
+        if (vpt == 2) {
+          if (v.stage) {
+            FRSZ2_DL(2, true, true, 64);
+          }
+          FRSZ2_DL(2, true, false, 64);
+        }
+      }
+      vpt = vpt < 4 ? 4 : vpt;
+      switch (vpt * 2 + v.stage) {
+        case 8: FRSZ2_DL(4, true, false, 64);
+        case 9: FRSZ2_DL(4, true, true, 64);
+        case 16: FRSZ2_DL(8, true, false, 64);
+        case 17: FRSZ2_DL(8, true, true, 64);
+        case 32: FRSZ2_DL(16, true, false, 64);
+        default: FRSZ2_DL(16, true, true, 64);
+      }
+    }
     if constexpr (sizeof(T) == 8 && U >= 16) {
       if (vpt == 2) {
         switch (v.stage * 2 + v.nt) {
-          case 0: FRSZ2_DL(2, false, false);
-          case 1: FRSZ2_DL(2, true, false);
-          case 2: FRSZ2_DL(2, false, true);
-          default: FRSZ2_DL(2, true, true);
+          case 0: FRSZ2_DL(2, false, false, WG);
+          case 1: FRSZ2_DL(2, true, false, WG);
+          case 2: FRSZ2_DL(2, false, true, WG);
+          default: FRSZ2_DL(2, true, true, WG);
         }
       }
     }
     switch (vpt * 4 + v.stage * 2 + v.nt) {
-      case 16: FRSZ2_DL(4, false, false);
-      case 17: FRSZ2_DL(4, true, false);
-      case 18: FRSZ2_DL(4, false, true);
-      case 19: FRSZ2_DL(4, true, true);
-      case 32: FRSZ2_DL(8, false, false);
-      case 33: FRSZ2_DL(8, true, false);
-      case 34: FRSZ2_DL(8, false, true);
-      case 35: FRSZ2_DL(8, true, true);
-      case 64: FRSZ2_DL(16, false, false);
-      case 65: FRSZ2_DL(16, true, false);
-      case 66: FRSZ2_DL(16, false, true);
-      default: FRSZ2_DL(16, true, true);
+      case 16: FRSZ2_DL(4, false, false, WG);
+      case 17: FRSZ2_DL(4, true, false, WG);
+      case 18: FRSZ2_DL(4, false, true, WG);
+      case 19: FRSZ2_DL(4, true, true, WG);
+      case 32: FRSZ2_DL(8, false, false, WG);
+      case 33: FRSZ2_DL(8, true, false, WG);
+      case 34: FRSZ2_DL(8, false, true, WG);
+      case 35: FRSZ2_DL(8, true, true, WG);
+      case 64: FRSZ2_DL(16, false, false, WG);
+      case 65: FRSZ2_DL(16, true, false, WG);
+      case 66: FRSZ2_DL(16, false, true, WG);
+      default: FRSZ2_DL(16, true, true, WG);
     }
 #undef FRSZ2_DL
   }

@@ -57,6 +57,14 @@ constexpr int lds_dwords()
   return WG * VPT * 2 + 4;
 }
 
+// the same for codes of at most U bits (a tile's payload is at most
+// WG*VPT*U/8 bytes)
+template<int U, int VPT, int WGS = WG>
+constexpr int lds_dwords_u()
+{
+  return WGS * VPT * U / 32 + 4;
+}
+
 // 128/64/32-bit global access, optionally with the non-temporal hint
 template<bool NT, typename V>
 __device__ __forceinline__ V
@@ -368,7 +376,7 @@ block_exponents(const std::uint32_t (&key)[VPT],
   }
 }
 
-template<typename T, int U, int VPT, bool PACKED, bool NT>
+template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS = WG>
 __device__ __forceinline__ void
 compress_tile(const typename fmt<T>::uw* __restrict__ in,
               std::uint8_t* __restrict__ out,
@@ -378,7 +386,7 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
               std::uint32_t* wred)
 {
   using uw = typename fmt<T>::uw;
-  constexpr int TILE = WG * VPT;
+  constexpr int TILE = WGS * VPT;
   const int t = static_cast<int>(threadIdx.x);
   const std::uint64_t base = tile * TILE;
   const std::uint64_t idx0 = base + static_cast<std::uint64_t>(t) * VPT;
@@ -429,7 +437,7 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
     const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
     const std::uint32_t tile_bytes = TB * p.CB;
     u32x4* l4 = reinterpret_cast<u32x4*>(lds);
-    for (std::uint32_t c = t; c < tile_bytes / 16; c += WG) {
+    for (std::uint32_t c = t; c < tile_bytes / 16; c += WGS) {
       l4[c] = u32x4{ 0u, 0u, 0u, 0u };
     }
     __syncthreads();
@@ -449,7 +457,7 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
     const std::uint32_t nbytes = blocks * p.CB;
     std::uint8_t* dst = pay + k0 * p.CB;
     u32x4* d4 = reinterpret_cast<u32x4*>(dst);
-    for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
+    for (std::uint32_t c = t; c < nbytes / 16; c += WGS) {
       st<NT>(l4[c], d4 + c);
     }
     const std::uint32_t tail = nbytes & 15u;
@@ -475,26 +483,26 @@ load_bytes(const std::uint8_t* __restrict__ src, std::uint32_t (&w)[NB >= 4 ? NB
 
 // Decompress one tile.  A thread's VPT values are split into NCH chunks of
 // VC consecutive values, chunk c of thread t holding values
-// [(c*WG + t)*VC, +VC) of the tile.  With codes read from global memory
-// (bits == U) a chunk is 16 bytes of codes (VC = 128/U, at most VPT), so
-// every code load is one 128-bit access and, when a thread has several
-// chunks, each load / store instruction of the workgroup covers one
-// contiguous span.  With codes staged in LDS (bits < U) a thread has one
-// chunk of VPT consecutive values.  All loads of a thread are issued before
-// any value is decoded.  With STAGE the decoded tile goes through LDS and
-// out with 16 bytes per lane per store instruction, so every instruction
-// writes whole contiguous cache lines (a lane's own values are 32+ bytes, so
-// storing them directly leaves each instruction half-filling its lines, and
-// L2 then writes some lines back twice).
+// [(c*WG + t)*VC, +VC) of the tile, so each load / store instruction of the
+// workgroup covers one contiguous span.  All loads of a thread are issued
+// before any value is decoded.
+//   direct (!STAGE): a chunk is 16 bytes of output (4 float32 / 2 float64),
+//     stored straight from registers; its codes are VC*U/8 bytes (bits == U)
+//     or extracted from the LDS image of the tile's payload (bits < U).
+//   STAGE: a chunk is 16 bytes of codes (bits == U; VC = 128/U, at most VPT)
+//     or all VPT values of the thread (bits < U); the decoded tile goes
+//     through LDS and out with 16 bytes per lane per store instruction.
+//     This keeps the code loads 128-bit wide when a chunk's output is 32+
+//     bytes per lane, without stores that half-fill their cache lines.
 // LDS dwords of the output staging buffer of a decompress tile
-template<typename T, int VPT>
+template<typename T, int VPT, int WGS = WG>
 constexpr int
 stage_dwords()
 {
-  return WG * VPT * static_cast<int>(sizeof(T)) / 4;
+  return WGS * VPT * static_cast<int>(sizeof(T)) / 4;
 }
 
-template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE>
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int WGS = WG>
 __device__ __forceinline__ void
 decompress_tile(const std::uint8_t* __restrict__ in,
                 typename fmt<T>::uw* __restrict__ out,
@@ -504,8 +512,13 @@ decompress_tile(const std::uint8_t* __restrict__ in,
                 std::uint32_t* stage)
 {
   using uw = typename fmt<T>::uw;
-  constexpr int TILE = WG * VPT;
-  constexpr int VC = PACKED ? VPT : (VPT < 128 / U ? VPT : 128 / U);
+  constexpr int TILE = WGS * VPT;
+  // values per chunk.  Staged: 16 bytes of codes (or all of a thread's
+  // values when the codes come from the LDS image), restaged through LDS for
+  // whole-line stores.  Direct: one 16-byte output store per chunk, so each
+  // store instruction of a wave covers 1 KiB of consecutive output.
+  constexpr int OUTV = 16 / static_cast<int>(sizeof(T));
+  constexpr int VC = STAGE ? (PACKED ? VPT : (VPT < 128 / U ? VPT : 128 / U)) : (VPT < OUTV ? VPT : OUTV);
   constexpr int NCH = VPT / VC;
   constexpr int NB = VC * U / 8; // code bytes per chunk
   static_assert(VC * static_cast<int>(sizeof(T)) % 16 == 0, "a chunk is whole 16-byte stores");
@@ -514,33 +527,13 @@ decompress_tile(const std::uint8_t* __restrict__ in,
   const int B = 1 << p.lb;
   const std::uint8_t* pay = in + p.R;
 
-  if (PACKED) {
-    const std::uint64_t k0 = base >> p.lb;
-    const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
-    const std::uint64_t left = p.nb - k0;
-    const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
-    const std::uint32_t nbytes = blocks * p.CB;
-    const std::uint8_t* src = pay + k0 * p.CB;
-    const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
-    u32x4* l4 = reinterpret_cast<u32x4*>(lds);
-    for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
-      l4[c] = ld<NT>(s4 + c);
-    }
-    const std::uint32_t tail = nbytes & 15u;
-    if (static_cast<std::uint32_t>(t) < tail) {
-      const std::uint32_t o = nbytes - tail + t;
-      reinterpret_cast<std::uint8_t*>(lds)[o] = src[o];
-    }
-    __syncthreads();
-  }
-
-  const std::uint64_t limit = p.nb << p.lb; // payload words
+  // block exponents first: they do not depend on the payload, so their loads
+  // are in flight together with the payload loads (and, for packed codes,
+  // before the barrier that publishes the staged payload)
   int E[NCH][VC];
-  uw code[NCH][VC];
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
-    const std::uint32_t local0 = static_cast<std::uint32_t>((c * WG + t) * VC);
-    const std::uint64_t i0 = base + local0;
+    const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WGS + t) * VC);
     if (B >= VC) {
       const int e = i0 < p.n ? load_block_exponent<U>(in, i0 >> p.lb) : 0;
 #pragma unroll
@@ -553,6 +546,34 @@ decompress_tile(const std::uint8_t* __restrict__ in,
         E[c][v] = (i0 + v < p.n) ? load_block_exponent<U>(in, (i0 + v) >> p.lb) : 0;
       }
     }
+  }
+
+  if (PACKED) {
+    const std::uint64_t k0 = base >> p.lb;
+    const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
+    const std::uint64_t left = p.nb - k0;
+    const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
+    const std::uint32_t nbytes = blocks * p.CB;
+    const std::uint8_t* src = pay + k0 * p.CB;
+    const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
+    u32x4* l4 = reinterpret_cast<u32x4*>(lds);
+    for (std::uint32_t c = t; c < nbytes / 16; c += WGS) {
+      l4[c] = ld<NT>(s4 + c);
+    }
+    const std::uint32_t tail = nbytes & 15u;
+    if (static_cast<std::uint32_t>(t) < tail) {
+      const std::uint32_t o = nbytes - tail + t;
+      reinterpret_cast<std::uint8_t*>(lds)[o] = src[o];
+    }
+    __syncthreads();
+  }
+
+  const std::uint64_t limit = p.nb << p.lb; // payload words
+  uw code[NCH][VC];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    const std::uint32_t local0 = static_cast<std::uint32_t>((c * WGS + t) * VC);
+    const std::uint64_t i0 = base + local0;
     if (!PACKED) {
       if (i0 + VC <= limit) {
         std::uint32_t w[NB >= 4 ? NB / 4 : 1];
@@ -587,7 +608,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     uw* sv = reinterpret_cast<uw*>(stage);
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-      const std::uint32_t local0 = static_cast<std::uint32_t>((c * WG + t) * VC);
+      const std::uint32_t local0 = static_cast<std::uint32_t>((c * WGS + t) * VC);
       uw y[VC];
 #pragma unroll
       for (int v = 0; v < VC; ++v) {
@@ -603,7 +624,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     const u32x4* s4 = reinterpret_cast<const u32x4*>(stage);
     u32x4* o4 = reinterpret_cast<u32x4*>(out + base);
 #pragma unroll
-    for (int q0 = 0; q0 < NQ; q0 += WG) {
+    for (int q0 = 0; q0 < NQ; q0 += WGS) {
       const std::uint32_t q = static_cast<std::uint32_t>(q0 + t);
       if ((q + 1) * CPV <= nvals) {
         st<NT>(s4[q], o4 + q);
@@ -617,7 +638,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
   } else {
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-      const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WG + t) * VC);
+      const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WGS + t) * VC);
       uw y[VC];
 #pragma unroll
       for (int v = 0; v < VC; ++v) {

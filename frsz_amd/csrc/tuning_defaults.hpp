@@ -1,18 +1,20 @@
 // Default kernel variants of the tiled kernels, measured on MI355X (1 GiB
-// float64 field; 10^8-value float32 / float64 arrays), shared by the tuning
-// table of the plain kernels and the grouped batch kernels.
+// float64 field; 10^8-value float32 and 5*10^7-value float64 arrays of the
+// edge-case generator), shared by the tuning table of the plain kernels and
+// the grouped batch kernels (which always run 256-thread workgroups).
 #pragma once
 
 namespace frsz2 {
 
 // encoded variant: values per thread, staged decompress stores, non-temporal
-// hint, linear (workgroup-order) tiles
+// hint, linear (workgroup-order) tiles, 64-thread workgroups (plain kernels)
 constexpr int
-venc(int vpt, int stage, int nt, int linear = 0)
+venc(int vpt, int stage, int nt, int linear = 0, int small = 0)
 {
-  return vpt * 8 + linear * 4 + stage * 2 + nt;
+  return vpt * 16 + small * 8 + linear * 4 + stage * 2 + nt;
 }
-constexpr int venc_vpt(int v) { return v / 8; }
+constexpr int venc_vpt(int v) { return v / 16; }
+constexpr int venc_small(int v) { return (v / 8) % 2; }
 constexpr int venc_linear(int v) { return (v / 4) % 2; }
 constexpr int venc_stage(int v) { return (v / 2) % 2; }
 constexpr int venc_nt(int v) { return v % 2; }
@@ -25,22 +27,25 @@ default_variant(int op, int dtype, int ui, int packed)
   if (op == 0) {
     if (dtype == 0) {
       if (packed) {
-        return ui == 3 ? venc(4, 0, 1, 1) : venc(8, 0, 0, 0);
+        return ui == 2 ? venc(8, 0, 1, 1) : venc(16, 0, 0, 1);
       }
-      return ui == 0 ? venc(8, 0, 0, 1) : ui == 1 ? venc(8, 0, 1, 1) : venc(4, 0, 1, 1);
+      return ui == 0 ? venc(8, 0, 0, 1) : ui == 1 ? venc(8, 0, 1, 1) : venc(4, 0, 1, 1, 1);
     }
     if (packed) {
-      return venc(4, 0, 1, 0);
+      return ui <= 1 ? venc(4, 0, 0, 1) : venc(4, 0, 1, 1);
     }
-    return ui == 1 ? venc(4, 0, 1, 0) : venc(4, 0, 1, 1);
+    return ui == 1 ? venc(4, 0, 1, 0) : ui == 3 ? venc(2, 0, 1, 1, 1) : venc(4, 0, 1, 1);
   }
   if (dtype == 0) {
     if (packed) {
-      return ui == 3 ? venc(4, 0, 1, 1) : venc(8, 0, 1, 1);
+      return ui == 2 ? venc(8, 1, 1, 1, 1) : venc(16, 1, 1, 1, 1);
     }
-    return ui <= 1 ? venc(16, 1, 1, 1) : ui == 2 ? venc(8, 0, 1, 1) : venc(4, 0, 1, 1);
+    return ui == 1 ? venc(16, 1, 1, 1, 1) : venc(16, 1, 1, 1);
   }
-  return packed ? venc(4, 0, 1, 1) : venc(8, 1, 1, 1);
+  if (packed) {
+    return ui == 3 ? venc(4, 1, 1, 1, 1) : venc(8, 1, 1, 1);
+  }
+  return venc(8, 1, 1, 1);
 }
 
 } // namespace frsz2

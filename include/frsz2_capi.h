@@ -164,14 +164,17 @@ int frsz2_decompress_blocks(int dtype,
  * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) writes each tile's output
  * through LDS with whole-cache-line stores, FRSZ2_TUNE_LINEAR_ORDER takes
  * tiles in workgroup order (default: one contiguous range of tiles per
- * XCD).  Process-wide.  No counterpart
+ * XCD), FRSZ2_TUNE_SMALL_WORKGROUPS launches 64-thread workgroups (one wave,
+ * tiles of 64 x values_per_thread; implies the non-temporal hint) instead of
+ * 256-thread ones.  Process-wide.  No counterpart
  * in the reference (its kernels have a fixed launch shape,
  * test/test_frsz_gpu.cu:325-330). */
 enum
 {
   FRSZ2_TUNE_NONTEMPORAL = 1,
   FRSZ2_TUNE_STAGED_STORES = 2,
-  FRSZ2_TUNE_LINEAR_ORDER = 4
+  FRSZ2_TUNE_LINEAR_ORDER = 4,
+  FRSZ2_TUNE_SMALL_WORKGROUPS = 8
 };
 int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags);
 int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags);

@@ -1,5 +1,6 @@
 """Every tuned variant of the plain kernels (values per thread 2/4/8/16,
-non-temporal hint, staged decompress stores, tile order) against the oracle:
+non-temporal hint, staged decompress stores, tile order, 64- or 256-thread
+workgroups) against the oracle:
 the defaults can change with tuning, so all variants must be bit-exact."""
 import itertools
 
@@ -39,16 +40,16 @@ def test_all_variants(fz, port, dtype, bits, B):
     tdt = torch.float32 if dtype == np.float32 else torch.float64
     saved = [_capi.tune_get(op, code, bits) for op in (0, 1)]
     try:
-        for op, vpt, nt, st, lin in itertools.product((0, 1), (2, 4, 8, 16), (0, 1), (0, 1), (0, 1)):
-            if op == 0 and st:
-                continue
-            _capi.tune_set(op, code, bits, vpt, nt, st, lin)
+        for op, vpt, nt, st, lin, sm in itertools.product((0, 1), (2, 4, 8, 16), (0, 1), (0, 1), (0, 1), (0, 1)):
+            if (op == 0 and st) or (sm and not nt):
+                continue  # staged stores: decompress only; 64-thread workgroups imply the non-temporal hint
+            _capi.tune_set(op, code, bits, vpt, nt, st, lin, sm)
             if op == 0:
                 got = fz.compress(xd, bits, B).cpu().numpy()
-                assert np.array_equal(got, want), ("compress", vpt, nt, lin)
+                assert np.array_equal(got, want), ("compress", vpt, nt, lin, sm)
             else:
                 y = fz.decompress(torch.from_numpy(want).cuda(), n, tdt, bits, B).cpu().numpy()
-                assert np.array_equal(y.view(np.uint8), want_y.view(np.uint8)), ("decompress", vpt, nt, st, lin)
+                assert np.array_equal(y.view(np.uint8), want_y.view(np.uint8)), ("decompress", vpt, nt, st, lin, sm)
     finally:
-        for op, (v, nt, st, lin) in zip((0, 1), saved):
-            _capi.tune_set(op, code, bits, v, nt, st, lin)
+        for op, cfg in zip((0, 1), saved):
+            _capi.tune_set(op, code, bits, *cfg)
