@@ -134,6 +134,14 @@ class_vpt(int cls, int shape)
   return packed ? 4 : 8;
 }
 
+// threads per workgroup of a class's tiles: 64 where the grouped shape runs
+// a plain-kernel default that uses one-wave workgroups, else 256
+__host__ __device__ constexpr int
+class_wgs(int cls, int shape)
+{
+  return shape == 3 && cls_op(cls) != 2 && venc_small(cls_default(cls)) != 0 ? 64 : WG;
+}
+
 __host__ __device__ constexpr bool
 class_stage(int cls, int shape)
 {
@@ -323,12 +331,12 @@ k_reset_counter(unsigned long long* counter)
 
 // Grouped mode: one workgroup per tile of the jobs of one class (jobs[0..njobs)
 // are that class's descriptors, tiles tile_base + blockIdx.x).
-template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP>
-__global__ __launch_bounds__(WG) void k_grouped(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base)
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
+__global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base)
 {
-  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT>() : 1];
-  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT>() : 1];
-  __shared__ std::uint32_t wred[WG / tile::WAVE];
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
+  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
+  __shared__ std::uint32_t wred[WGS / tile::WAVE];
   const std::uint64_t g = tile_base + blockIdx.x;
   const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, 0));
   const bjob J = jobs[j];
@@ -339,15 +347,15 @@ __global__ __launch_bounds__(WG) void k_grouped(const bjob* __restrict__ jobs, i
     const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb < 0 ? 0 : J.lb, 0 };
     if constexpr (OP == 0) {
       auto* o8 = static_cast<std::uint8_t*>(J.out);
-      tile::compress_tile<T, U, VPT, PACKED, NT>(static_cast<const typename fmt<T>::uw*>(J.in), o8, p, t, lds,
-                                                 wred);
+      tile::compress_tile<T, U, VPT, PACKED, NT, WGS>(static_cast<const typename fmt<T>::uw*>(J.in), o8, p, t, lds,
+                                                      wred);
       if (t == 0) {
-        tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);
+        tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WGS);
       }
     } else {
-      tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE>(static_cast<const std::uint8_t*>(J.in),
-                                                          static_cast<typename fmt<T>::uw*>(J.out), p, t, lds,
-                                                          stage);
+      tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE, WGS>(static_cast<const std::uint8_t*>(J.in),
+                                                               static_cast<typename fmt<T>::uw*>(J.out), p, t, lds,
+                                                               stage);
     }
   }
 }
@@ -364,8 +372,9 @@ launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, hipStrea
   constexpr bool PACKED = cls_packed(CLS) != 0;
   constexpr bool NT = venc_nt(V) != 0;
   constexpr bool STAGE = OP == 1 && venc_stage(V) != 0;
-  hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP>), dim3(static_cast<unsigned>(nt)), dim3(WG), 0,
-                     s, d, nd, t0);
+  constexpr int WGS = class_wgs(CLS, 3);
+  hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)), dim3(WGS),
+                     0, s, d, nd, t0);
 }
 
 int
@@ -486,7 +495,8 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
     if (j.op == FRSZ2_OP_GATHER) {
       nt = (j.m + GATHER_TILE - 1) / GATHER_TILE;
     } else {
-      const std::uint64_t TILE = static_cast<std::uint64_t>(WG) * class_vpt(cls, tile_shape);
+      const std::uint64_t TILE =
+        static_cast<std::uint64_t>(class_wgs(cls, tile_shape)) * class_vpt(cls, tile_shape);
       if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
         continue;
       }
