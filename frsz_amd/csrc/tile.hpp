@@ -237,18 +237,46 @@ store_exponent(std::uint8_t* __restrict__ out, std::uint64_t k, int e)
   store_block_exponent<U>(out, k, e);
 }
 
+// max over aligned groups of G lanes (G a power of two up to the wave)
+template<int G>
+__device__ __forceinline__ std::uint32_t
+lane_max(std::uint32_t v)
+{
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const std::uint32_t o = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off));
+    v = v > o ? v : o;
+  }
+  return v;
+}
+
 // max over the G lanes of an aligned lane group (G a power of two); for
 // G > 64 the wave maxima are combined through wred (one dword per wave).
 __device__ __forceinline__ std::uint32_t
 group_max(std::uint32_t v, int G, std::uint32_t* wred)
 {
-  const int gw = G < WAVE ? G : WAVE;
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    if (off < gw) {
-      const std::uint32_t o = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off));
-      v = v > o ? v : o;
-    }
+  // G is uniform: one branch picks a butterfly with compile-time lane masks
+  switch (G < WAVE ? G : WAVE) {
+    case 1:
+      break;
+    case 2:
+      v = lane_max<2>(v);
+      break;
+    case 4:
+      v = lane_max<4>(v);
+      break;
+    case 8:
+      v = lane_max<8>(v);
+      break;
+    case 16:
+      v = lane_max<16>(v);
+      break;
+    case 32:
+      v = lane_max<32>(v);
+      break;
+    default:
+      v = lane_max<64>(v);
+      break;
   }
   if (G > WAVE) {
     const int wave = threadIdx.x / WAVE;

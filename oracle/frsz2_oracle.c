@@ -21,6 +21,9 @@
  *       by the compiled reference header (tests/golden/), the reference's own
  *       round-trip cases (test/test_frsz_gpu.cu:382-441), and invariants
  *       (lossless at b = W, B = 1; truncation error bound; block locality).
+ *   fo_compress_block / fo_decompress_block -- the compiled reference's own
+ *       compress_cpu_block / decompress_cpu_block on random blocks, lengths
+ *       (partial blocks inside the array included) and buffer contents.
  *   No function is left unpinned.
  *
  * Behaviour that is undefined in the reference (shift counts >= width, which
