@@ -340,70 +340,112 @@ extract_code(const std::uint32_t* lds, std::uint32_t pos, int bits)
   }
 }
 
-// Per-value block exponents for one thread's VPT values.  Also stores the
-// exponents of the blocks that start at this thread's values.
-template<typename T, int U, int VPT>
+// Block exponents of a thread's NCH chunks of VC values (chunk c of thread t
+// holds tile values (c*WGS + t)*VC .. +VC), per value in E.  The lane that
+// holds a block's first value stores the block's exponent.
+template<typename T, int U, int VC, int NCH, int WGS>
 __device__ __forceinline__ void
-block_exponents(const std::uint32_t (&key)[VPT],
-                int (&E)[VPT],
-                std::uint64_t idx0,
+chunk_exponents(const std::uint32_t (&key)[NCH][VC],
+                int (&E)[NCH][VC],
+                std::uint64_t base,
                 const tplan& p,
                 std::uint8_t* __restrict__ out,
                 std::uint32_t* wred)
 {
   const int B = 1 << p.lb;
-  if (B >= VPT) {
-    std::uint32_t m = key[0];
+  const int t = static_cast<int>(threadIdx.x);
+  if (B >= WGS * VC) {
+    // a block covers R whole chunk rows of the workgroup
+    const int R = B / (WGS * VC);
 #pragma unroll
-    for (int v = 1; v < VPT; ++v) {
-      m = m > key[v] ? m : key[v];
-    }
-    m = group_max(m, B / VPT, wred);
-    const int e = block_exponent_from_key<T, U>(m);
+    for (int r0 = 0; r0 < NCH; ++r0) {
+      if ((r0 & (R - 1)) == 0) {
+        std::uint32_t m = 0;
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      E[v] = e;
+        for (int c = 0; c < NCH; ++c) {
+          if (c >= r0 && c < r0 + R) {
+#pragma unroll
+            for (int v = 0; v < VC; ++v) {
+              m = m > key[c][v] ? m : key[c][v];
+            }
+          }
+        }
+        m = group_max(m, WGS, wred);
+        const int e = block_exponent_from_key<T, U>(m);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          if (c >= r0 && c < r0 + R) {
+#pragma unroll
+            for (int v = 0; v < VC; ++v) {
+              E[c][v] = e;
+            }
+          }
+        }
+        const std::uint64_t k = (base + static_cast<std::uint64_t>(r0) * WGS * VC) >> p.lb;
+        if (t == 0 && k < p.nb) {
+          store_exponent<U>(out, k, e);
+        }
+      }
     }
-    if ((idx0 & static_cast<std::uint64_t>(B - 1)) == 0) {
-      const std::uint64_t k = idx0 >> p.lb;
-      if (k < p.nb) {
-        store_exponent<U>(out, k, e);
+  } else if (B >= VC) {
+    // a block covers B/VC lanes of one chunk row
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      std::uint32_t m = key[c][0];
+#pragma unroll
+      for (int v = 1; v < VC; ++v) {
+        m = m > key[c][v] ? m : key[c][v];
+      }
+      m = group_max(m, B / VC, wred);
+      const int e = block_exponent_from_key<T, U>(m);
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        E[c][v] = e;
+      }
+      const std::uint64_t i0 = base + static_cast<std::uint64_t>(c * WGS + t) * VC;
+      if ((i0 & static_cast<std::uint64_t>(B - 1)) == 0 && (i0 >> p.lb) < p.nb) {
+        store_exponent<U>(out, i0 >> p.lb, e);
       }
     }
   } else {
-    // B < VPT: whole blocks inside the thread; butterfly max over runs of B
-    std::uint32_t m[VPT];
+    // B < VC: whole blocks inside a chunk; butterfly max over runs of B
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      m[v] = key[v];
-    }
+    for (int c = 0; c < NCH; ++c) {
+      std::uint32_t m[VC];
 #pragma unroll
-    for (int s = 1; s < VPT; s <<= 1) {
-      if (s < B) {
-        std::uint32_t t[VPT];
+      for (int v = 0; v < VC; ++v) {
+        m[v] = key[c][v];
+      }
 #pragma unroll
-        for (int v = 0; v < VPT; ++v) {
-          t[v] = m[v] > m[v ^ s] ? m[v] : m[v ^ s];
-        }
+      for (int s = 1; s < VC; s <<= 1) {
+        if (s < B) {
+          std::uint32_t q[VC];
 #pragma unroll
-        for (int v = 0; v < VPT; ++v) {
-          m[v] = t[v];
+          for (int v = 0; v < VC; ++v) {
+            q[v] = m[v] > m[v ^ s] ? m[v] : m[v ^ s];
+          }
+#pragma unroll
+          for (int v = 0; v < VC; ++v) {
+            m[v] = q[v];
+          }
         }
       }
-    }
+      const std::uint64_t i0 = base + static_cast<std::uint64_t>(c * WGS + t) * VC;
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      E[v] = block_exponent_from_key<T, U>(m[v]);
-      if ((v & (B - 1)) == 0) {
-        const std::uint64_t k = (idx0 + v) >> p.lb;
-        if (k < p.nb) {
-          store_exponent<U>(out, k, E[v]);
+      for (int v = 0; v < VC; ++v) {
+        E[c][v] = block_exponent_from_key<T, U>(m[v]);
+        if ((v & (B - 1)) == 0 && ((i0 + v) >> p.lb) < p.nb) {
+          store_exponent<U>(out, (i0 + v) >> p.lb, E[c][v]);
         }
       }
     }
   }
 }
 
+// Compress one tile.  A thread's VPT values are NCH chunks of VC = 16 bytes
+// of input (4 float32 / 2 float64), chunk c of thread t holding tile values
+// [(c*WGS + t)*VC, +VC), so every load instruction of a wave reads 1 KiB of
+// consecutive input and every code store writes consecutive bytes.
 template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS = WG>
 __device__ __forceinline__ void
 compress_tile(const typename fmt<T>::uw* __restrict__ in,
@@ -415,47 +457,63 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
 {
   using uw = typename fmt<T>::uw;
   constexpr int TILE = WGS * VPT;
+  constexpr int VC = 16 / static_cast<int>(sizeof(T));
+  constexpr int NCH = VPT / VC;
+  static_assert(VPT % VC == 0, "a thread holds whole 16-byte chunks");
   const int t = static_cast<int>(threadIdx.x);
   const std::uint64_t base = tile * TILE;
-  const std::uint64_t idx0 = base + static_cast<std::uint64_t>(t) * VPT;
 
-  uw x[VPT];
-  if (idx0 + VPT <= p.n) {
-    load_values<T, VPT, NT>(in + idx0, x);
-  } else {
+  uw x[NCH][VC];
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      x[v] = (idx0 + v < p.n) ? in[idx0 + v] : uw{ 0 };
+  for (int c = 0; c < NCH; ++c) {
+    const std::uint64_t i0 = base + static_cast<std::uint64_t>(c * WGS + t) * VC;
+    if (i0 + VC <= p.n) {
+      load_values<T, VC, NT>(in + i0, x[c]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        x[c][v] = (i0 + v < p.n) ? in[i0 + v] : uw{ 0 };
+      }
     }
   }
-  std::uint32_t key[VPT];
+  std::uint32_t key[NCH][VC];
 #pragma unroll
-  for (int v = 0; v < VPT; ++v) {
-    key[v] = max_key<T>(x[v]);
+  for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+    for (int v = 0; v < VC; ++v) {
+      key[c][v] = max_key<T>(x[c][v]);
+    }
   }
-  int E[VPT];
-  block_exponents<T, U, VPT>(key, E, idx0, p, out, wred);
+  int E[NCH][VC];
+  chunk_exponents<T, U, VC, NCH, WGS>(key, E, base, p, out, wred);
 
-  uw code[VPT];
+  uw code[NCH][VC];
 #pragma unroll
-  for (int v = 0; v < VPT; ++v) {
-    code[v] = encode_value<T>(x[v], E[v], p.bits);
+  for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+    for (int v = 0; v < VC; ++v) {
+      code[c][v] = encode_value<T>(x[c][v], E[c][v], p.bits);
+    }
   }
 
   std::uint8_t* pay = out + p.R;
   if (!PACKED) {
     // bits == U: payload word i holds value i; words [n, nb*B) get zero codes
+    using wt = typename word<U>::type;
     const std::uint64_t limit = p.nb << p.lb;
-    if (idx0 + VPT <= limit) {
-      std::uint32_t w[VPT * U / 32 > 0 ? VPT * U / 32 : 1];
-      pack_words<U, VPT>(code, w);
-      store_dwords<NT>(pay + idx0 * (U / 8), w);
-    } else {
-      using wt = typename word<U>::type;
 #pragma unroll
-      for (int v = 0; v < VPT; ++v) {
-        if (idx0 + v < limit) {
-          reinterpret_cast<wt*>(pay)[idx0 + v] = static_cast<wt>(code[v]);
+    for (int c = 0; c < NCH; ++c) {
+      const std::uint64_t i0 = base + static_cast<std::uint64_t>(c * WGS + t) * VC;
+      if (VC * U >= 32 && i0 + VC <= limit) {
+        std::uint32_t w[VC * U / 32 > 0 ? VC * U / 32 : 1];
+        pack_words<U, VC>(code[c], w);
+        store_dwords<NT>(pay + i0 * (U / 8), w);
+      } else {
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          if (i0 + v < limit) {
+            reinterpret_cast<wt*>(pay)[i0 + v] = static_cast<wt>(code[c][v]);
+          }
         }
       }
     }
@@ -470,13 +528,16 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
     }
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) {
-      if (code[v] != 0) {
-        const std::uint32_t local = static_cast<std::uint32_t>(t * VPT + v);
-        const std::uint32_t kl = local >> p.lb;
-        const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
-        const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
-        or_code<U>(lds, pos, static_cast<std::uint64_t>(code[v]), p.bits);
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        if (code[c][v] != 0) {
+          const std::uint32_t local = static_cast<std::uint32_t>((c * WGS + t) * VC + v);
+          const std::uint32_t kl = local >> p.lb;
+          const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
+          const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
+          or_code<U>(lds, pos, static_cast<std::uint64_t>(code[c][v]), p.bits);
+        }
       }
     }
     __syncthreads();
