@@ -237,14 +237,45 @@ store_exponent(std::uint8_t* __restrict__ out, std::uint64_t k, int e)
   store_block_exponent<U>(out, k, e);
 }
 
-// max over aligned groups of G lanes (G a power of two up to the wave)
+// max of v and the value of the lane that DPP control CTRL selects (a lane of
+// the same 16-lane row)
+template<int CTRL>
+__device__ __forceinline__ std::uint32_t
+dpp_max(std::uint32_t v)
+{
+  const std::uint32_t o =
+    static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+  return v > o ? v : o;
+}
+
+// max over aligned groups of G lanes (G a power of two up to the wave).
+// Within a row the steps are DPP moves fused into the max: quad_perm
+// [1,0,3,2] and [2,3,0,1] (each lane then holds its quad's max), row half
+// mirror (lane i <- 7-i: the other quad of the 8-lane group) and row mirror
+// (lane i <- 15-i: the other half of the row).  Groups of 32 and 64 lanes
+// finish with cross-row shuffles.
 template<int G>
 __device__ __forceinline__ std::uint32_t
 lane_max(std::uint32_t v)
 {
-#pragma unroll
-  for (int off = 1; off < G; off <<= 1) {
-    const std::uint32_t o = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off));
+  if constexpr (G >= 2) {
+    v = dpp_max<0xB1>(v);
+  }
+  if constexpr (G >= 4) {
+    v = dpp_max<0x4E>(v);
+  }
+  if constexpr (G >= 8) {
+    v = dpp_max<0x141>(v);
+  }
+  if constexpr (G >= 16) {
+    v = dpp_max<0x140>(v);
+  }
+  if constexpr (G >= 32) {
+    const std::uint32_t o = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), 16));
+    v = v > o ? v : o;
+  }
+  if constexpr (G >= 64) {
+    const std::uint32_t o = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), 32));
     v = v > o ? v : o;
   }
   return v;
