@@ -589,6 +589,34 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
   }
 }
 
+// Values of a chunk of VC codes.  `uniform`: the chunk lies in one exponent
+// block (E[0] holds for all of it, so the per-block terms are computed once);
+// `narrow`: bits <= SB + 1, where decode_value_short applies; both together
+// take the short path.  Both flags are the same for every thread of the
+// launch.
+template<typename T, int VC>
+__device__ __forceinline__ void
+decode_chunk(const typename fmt<T>::uw (&code)[VC],
+             const int (&E)[VC],
+             bool uniform,
+             bool narrow,
+             int bits,
+             typename fmt<T>::uw (&y)[VC])
+{
+  if (narrow && uniform) {
+    const int e = E[0];
+#pragma unroll
+    for (int v = 0; v < VC; ++v) {
+      y[v] = decode_value_short<T>(code[v], e, bits);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < VC; ++v) {
+      y[v] = decode_value<T>(code[v], E[v], bits);
+    }
+  }
+}
+
 // NB-byte code load / value store helpers of the chunked decompress
 template<bool NT, int NB>
 __device__ __forceinline__ void
@@ -689,6 +717,8 @@ decompress_tile(const std::uint8_t* __restrict__ in,
   }
 
   const std::uint64_t limit = p.nb << p.lb; // payload words
+  const bool uniform = B >= VC;              // one exponent per chunk
+  const bool narrow = p.bits <= fmt<T>::SB + 1;
   uw code[NCH][VC];
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
@@ -730,10 +760,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     for (int c = 0; c < NCH; ++c) {
       const std::uint32_t local0 = static_cast<std::uint32_t>((c * WGS + t) * VC);
       uw y[VC];
-#pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
-      }
+      decode_chunk<T, VC>(code[c], E[c], uniform, narrow, p.bits, y);
       store_values<T, VC, false>(sv + local0, y);
     }
     __syncthreads();
@@ -760,10 +787,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
     for (int c = 0; c < NCH; ++c) {
       const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WGS + t) * VC);
       uw y[VC];
-#pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        y[v] = decode_value<T>(code[c][v], E[c][v], p.bits);
-      }
+      decode_chunk<T, VC>(code[c], E[c], uniform, narrow, p.bits, y);
       if (i0 + VC <= p.n) {
         store_values<T, VC, NT>(out + i0, y);
       } else {

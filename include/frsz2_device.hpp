@@ -159,6 +159,30 @@ decode_value(typename fmt<T>::uw code, int E, int bits)
   return (s << (F::W - 1)) | ex | mant;
 }
 
+// decode_value for bits <= SB + 1 (float32 up to 24 bits, float64 up to 53)
+// and blocks with E + bias >= 0 (every block of finite data).  The
+// renormalising shift of the magnitude is then always to the left, by 1..SB
+// for a non-zero magnitude, so one shift replaces the two-way shift of
+// decode_value; the result is the same bit pattern.
+template<typename T>
+__host__ __device__ __forceinline__ typename fmt<T>::uw
+decode_value_short(typename fmt<T>::uw code, int E, int bits)
+{
+  using F = fmt<T>;
+  using uw = typename F::uw;
+  const uw cm = code & ((uw{ 1 } << (bits - 1)) - 1);
+  const uw top = cm << (F::W - bits);
+  // countl_zero(cm << (W - bits)), 32 for zero as the reference's host code
+  const int lz = top == 0 ? 32 : clz_nz(top);
+  const int eb = E + F::BIAS;
+  const int mz = (lz - 1) < eb ? (lz - 1) : eb;
+  const uw ex = static_cast<uw>(eb - mz) << F::SB;
+  const int lsh = ((F::W - bits) - (F::EB - 1) + mz) & (F::W - 1);
+  const uw mant = (cm << lsh) & ~(uw{ 1 } << F::SB);
+  const uw sgn = (code << (F::W - bits)) & (uw{ 1 } << (F::W - 1));
+  return sgn | ex | mant;
+}
+
 // Sign-extended exponent of block k (exponents start at the stream base).
 template<int U>
 __device__ __forceinline__ int

@@ -29,6 +29,39 @@ dm_roundtrip_f64(const std::uint64_t* u, std::uint64_t n, int E, int bits, std::
   }
 }
 
+// out[i] = decode_value_short(codes[i], E, bits) (the narrow-width decode)
+void
+dm_decode_short_f32(const std::uint32_t* codes, std::uint64_t n, int E, int bits, std::uint32_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::decode_value_short<float>(codes[i], E, bits);
+  }
+}
+
+void
+dm_decode_f32(const std::uint32_t* codes, std::uint64_t n, int E, int bits, std::uint32_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::decode_value<float>(codes[i], E, bits);
+  }
+}
+
+void
+dm_decode_short_f64(const std::uint64_t* codes, std::uint64_t n, int E, int bits, std::uint64_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::decode_value_short<double>(codes[i], E, bits);
+  }
+}
+
+void
+dm_decode_f64(const std::uint64_t* codes, std::uint64_t n, int E, int bits, std::uint64_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::decode_value<double>(codes[i], E, bits);
+  }
+}
+
 // block exponent (as stored) of a block whose values are u[0..n)
 int
 dm_block_exponent_f32(const std::uint32_t* u, std::uint64_t n, int U)
