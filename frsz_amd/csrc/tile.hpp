@@ -355,6 +355,54 @@ or_code(std::uint32_t* lds, std::uint32_t pos, std::uint64_t code, int bits)
   }
 }
 
+// OR the VC codes of one chunk into the LDS bit image when the chunk lies in
+// one exponent block: its codes are then consecutive in the bitstream, from
+// bit pos0 on.  They are assembled into one 128-bit value (VC * bits <= 128;
+// the offsets v * bits are the same for every lane, so the placement
+// branches are uniform), shifted by pos0 mod 32 and OR-ed in as the 1..5
+// dwords they touch: one LDS atomic per touched dword instead of one or two
+// per code.
+template<int VC, typename UW>
+__device__ __forceinline__ void
+or_chunk(std::uint32_t* lds, std::uint32_t pos0, const UW (&code)[VC], int bits)
+{
+  std::uint64_t lo = 0;
+  std::uint64_t hi = 0;
+#pragma unroll
+  for (int v = 0; v < VC; ++v) {
+    const std::uint64_t c = static_cast<std::uint64_t>(code[v]);
+    const int sh = v * bits;
+    if (sh == 0) {
+      lo |= c;
+    } else if (sh < 64) {
+      lo |= c << sh;
+      hi |= c >> (64 - sh);
+    } else {
+      hi |= c << (sh - 64);
+    }
+  }
+  const std::uint32_t q = pos0 >> 5;
+  const int o = static_cast<int>(pos0 & 31u);
+  // (hi:lo) << o as three 64-bit words; (x >> 1) >> (63 - o) is x >> (64 - o), and 0 for o == 0
+  const std::uint64_t w0 = lo << o;
+  const std::uint64_t w1 = (hi << o) | ((lo >> 1) >> (63 - o));
+  const std::uint64_t w2 = (hi >> 1) >> (63 - o);
+  const int nd = (o + VC * bits + 31) >> 5;
+  atomicOr(lds + q, static_cast<std::uint32_t>(w0));
+  if (nd > 1) {
+    atomicOr(lds + q + 1, static_cast<std::uint32_t>(w0 >> 32));
+  }
+  if (nd > 2) {
+    atomicOr(lds + q + 2, static_cast<std::uint32_t>(w1));
+  }
+  if (nd > 3) {
+    atomicOr(lds + q + 3, static_cast<std::uint32_t>(w1 >> 32));
+  }
+  if (nd > 4) {
+    atomicOr(lds + q + 4, static_cast<std::uint32_t>(w2));
+  }
+}
+
 template<int U>
 __device__ __forceinline__ std::uint64_t
 extract_code(const std::uint32_t* lds, std::uint32_t pos, int bits)
@@ -518,12 +566,23 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
   int E[NCH][VC];
   chunk_exponents<T, U, VC, NCH, WGS>(key, E, base, p, out, wred);
 
+  // narrow widths with one exponent per chunk: encode_value_short, with the
+  // per-block terms computed once per chunk (both flags are uniform)
+  const bool short_codes = p.bits <= fmt<T>::SB + 1 && (1 << p.lb) >= VC;
   uw code[NCH][VC];
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
+    if (short_codes) {
+      const int e = E[c][0];
 #pragma unroll
-    for (int v = 0; v < VC; ++v) {
-      code[c][v] = encode_value<T>(x[c][v], E[c][v], p.bits);
+      for (int v = 0; v < VC; ++v) {
+        code[c][v] = encode_value_short<T>(x[c][v], e, p.bits);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        code[c][v] = encode_value<T>(x[c][v], E[c][v], p.bits);
+      }
     }
   }
 
@@ -558,16 +617,27 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
       l4[c] = u32x4{ 0u, 0u, 0u, 0u };
     }
     __syncthreads();
+    if (B >= VC) {
+      // each chunk lies in one block
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
+      for (int c = 0; c < NCH; ++c) {
+        const std::uint32_t local = static_cast<std::uint32_t>((c * WGS + t) * VC);
+        const std::uint32_t kl = local >> p.lb;
+        const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
+        or_chunk<VC>(lds, kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits), code[c], p.bits);
+      }
+    } else {
 #pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        if (code[c][v] != 0) {
-          const std::uint32_t local = static_cast<std::uint32_t>((c * WGS + t) * VC + v);
-          const std::uint32_t kl = local >> p.lb;
-          const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
-          const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
-          or_code<U>(lds, pos, static_cast<std::uint64_t>(code[c][v]), p.bits);
+      for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+          if (code[c][v] != 0) {
+            const std::uint32_t local = static_cast<std::uint32_t>((c * WGS + t) * VC + v);
+            const std::uint32_t kl = local >> p.lb;
+            const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
+            const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
+            or_code<U>(lds, pos, static_cast<std::uint64_t>(code[c][v]), p.bits);
+          }
         }
       }
     }

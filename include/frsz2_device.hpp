@@ -117,6 +117,28 @@ encode_value(typename fmt<T>::uw u, int E, int bits)
   return mag | (s << (bits - 1));
 }
 
+// encode_value for bits <= SB + 1 (float32 up to 24 bits, float64 up to 53):
+// the folded shift (W - bits) - shift is then never negative for a value of
+// the block (E >= exponent(x)), so only the right shift and its clamp
+// remain.  Same code as encode_value for every such value, NaN included.
+template<typename T>
+__host__ __device__ __forceinline__ typename fmt<T>::uw
+encode_value_short(typename fmt<T>::uw u, int E, int bits)
+{
+  using F = fmt<T>;
+  using uw = typename F::uw;
+  const std::uint32_t biased = static_cast<std::uint32_t>((u & F::ABS) >> F::SB);
+  int e = static_cast<int>(biased) - F::BIAS;
+  if (sizeof(T) == 4) {
+    e = static_cast<std::int8_t>(static_cast<std::uint8_t>(e));
+  }
+  const uw m = (u & ((uw{ 1 } << F::SB) - 1)) | (biased != 0 ? (uw{ 1 } << F::SB) : uw{ 0 });
+  const int tot = (F::W - bits) - (F::EB - 1) + (E - e);
+  const uw mag = m >> (tot < F::W - 1 ? tot : F::W - 1);
+  const uw s = u >> (F::W - 1);
+  return mag | (s << (bits - 1));
+}
+
 // count of leading zeros of a W-bit value known to be non-zero
 __host__ __device__ __forceinline__ int
 clz_nz(std::uint32_t v)

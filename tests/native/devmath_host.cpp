@@ -29,6 +29,39 @@ dm_roundtrip_f64(const std::uint64_t* u, std::uint64_t n, int E, int bits, std::
   }
 }
 
+// out[i] = encode_value_short(u[i], E, bits) / encode_value (the narrow-width encode)
+void
+dm_encode_short_f32(const std::uint32_t* u, std::uint64_t n, int E, int bits, std::uint32_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::encode_value_short<float>(u[i], E, bits);
+  }
+}
+
+void
+dm_encode_f32(const std::uint32_t* u, std::uint64_t n, int E, int bits, std::uint32_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::encode_value<float>(u[i], E, bits);
+  }
+}
+
+void
+dm_encode_short_f64(const std::uint64_t* u, std::uint64_t n, int E, int bits, std::uint64_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::encode_value_short<double>(u[i], E, bits);
+  }
+}
+
+void
+dm_encode_f64(const std::uint64_t* u, std::uint64_t n, int E, int bits, std::uint64_t* out)
+{
+  for (std::uint64_t i = 0; i < n; ++i) {
+    out[i] = frsz2::encode_value<double>(u[i], E, bits);
+  }
+}
+
 // out[i] = decode_value_short(codes[i], E, bits) (the narrow-width decode)
 void
 dm_decode_short_f32(const std::uint32_t* codes, std::uint64_t n, int E, int bits, std::uint32_t* out)

@@ -1,5 +1,6 @@
 """The value-level device arithmetic (include/frsz2_device.hpp: encode_value,
-decode_value, decode_value_short, max_key, block_exponent_from_key), compiled
+decode_value and their narrow-width forms encode_value_short and
+decode_value_short, max_key, block_exponent_from_key), compiled
 for the host into tests/native/libdevmath_host.so, against the oracle on the
 CPU: every float32 exponent E and every bits value over a sample of bit
 patterns per biased exponent (edges included), float64 on a stride of E.
@@ -33,7 +34,7 @@ def roundtrip(lib, name, u, E, bits, oracle):
     return c, v
 
 
-def decode(lib, name, codes, E, bits):
+def apply(lib, name, codes, E, bits):
     out = np.empty_like(codes)
     getattr(lib, name)(_vp(codes.ctypes.data), ctypes.c_uint64(codes.size), ctypes.c_int(E), ctypes.c_int(bits),
                        _vp(out.ctypes.data))
@@ -91,8 +92,8 @@ def test_short_decode_float32_equals_generic(dm):
             codes = np.concatenate([np.array([0, 1, top, top >> 1, 1 << (bits - 1), (1 << (bits - 1)) + 1],
                                              np.uint32), rng.integers(0, 1 << bits, 4096, dtype=np.uint32)])
         for E in range(-127, 128):
-            a = decode(dm, "dm_decode_f32", codes, E, bits)
-            b = decode(dm, "dm_decode_short_f32", codes, E, bits)
+            a = apply(dm, "dm_decode_f32", codes, E, bits)
+            b = apply(dm, "dm_decode_short_f32", codes, E, bits)
             assert np.array_equal(a, b), (bits, E)
 
 
@@ -105,6 +106,39 @@ def test_short_decode_float64_equals_generic(dm):
         for E in list(range(-1023, 1024, 11)) + [-1022, -1, 0, 1, 1023]:
             if bits <= 8 and not -128 <= E <= 127:
                 continue
-            a = decode(dm, "dm_decode_f64", codes, E, bits)
-            b = decode(dm, "dm_decode_short_f64", codes, E, bits)
+            a = apply(dm, "dm_decode_f64", codes, E, bits)
+            b = apply(dm, "dm_decode_short_f64", codes, E, bits)
             assert np.array_equal(a, b), (bits, E)
+
+
+def test_short_encode_float32_equals_generic(dm):
+    """encode_value_short (bits <= 24) against encode_value on every biased
+    exponent (NaN patterns included) for every block exponent E >= exponent(x)."""
+    u = f32_patterns(np.random.default_rng(4), 120)
+    nan = np.array([0x7FC00000, 0xFFC00001, 0x7F800001, 0x7FFFFFFF], np.uint32)
+    exps = ((u >> np.uint32(23)) & np.uint32(0xFF)).astype(np.int64) - 127
+    for E in range(-127, 128):
+        sel = np.ascontiguousarray(np.concatenate([u[exps <= E], nan]))
+        for bits in range(4, 25):
+            a = apply(dm, "dm_encode_f32", sel, E, bits)
+            b = apply(dm, "dm_encode_short_f32", sel, E, bits)
+            assert np.array_equal(a, b), (bits, E)
+
+
+def test_short_encode_float64_equals_generic(dm):
+    rng = np.random.default_rng(5)
+    mant = np.concatenate([np.array([0, 1, (1 << 52) - 1, 1 << 51], np.uint64),
+                           rng.integers(0, 1 << 52, 40, dtype=np.uint64)])
+    be = np.arange(0, 2047, 5, dtype=np.uint64)
+    u = (be[:, None] << np.uint64(52) | mant[None, :]).ravel()
+    u = np.concatenate([u, u | np.uint64(1 << 63)])
+    exps = ((u >> np.uint64(52)) & np.uint64(0x7FF)).astype(np.int64) - 1023
+    for E in list(range(-1023, 1024, 41)) + [-1022, 0, 1023]:
+        sel = np.ascontiguousarray(u[exps <= E])
+        for bits in range(4, 54):
+            if bits <= 8 and not -128 <= E <= 127:
+                continue
+            a = apply(dm, "dm_encode_f64", sel, E, bits)
+            b = apply(dm, "dm_encode_short_f64", sel, E, bits)
+            assert np.array_equal(a, b), (bits, E)
+
