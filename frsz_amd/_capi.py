@@ -67,6 +67,9 @@ lib.frsz2_tune_set.restype = _int
 lib.frsz2_tune_set.argtypes = [_int, _int, _int, _int, _int]
 lib.frsz2_tune_get.restype = _int
 lib.frsz2_tune_get.argtypes = [_int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int)]
+lib.frsz2_tune_auto.restype = _int
+lib.frsz2_tune_auto.argtypes = [_int, _int, _int, _u64, _u64, _vp, _u64, _vp, _u64, _int, ctypes.POINTER(_int),
+                                ctypes.POINTER(_int), _vp]
 lib.frsz2_compress_host.restype = _int
 lib.frsz2_compress_host.argtypes = [_int, _int, _u64, _vp, _u64, _vp, _u64]
 lib.frsz2_decompress_host.restype = _int

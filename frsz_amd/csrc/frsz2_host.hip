@@ -146,6 +146,100 @@ frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags)
 }
 
 int
+frsz2_tune_auto(int op, int dtype, int bits, uint64_t block, uint64_t n, const void* d_in, uint64_t in_bytes,
+                void* d_out, uint64_t out_bytes, int reps, int* values_per_thread, int* flags, frsz2_stream_t stream)
+{
+  using namespace frsz2;
+  clear_error();
+  int rc = check_params(dtype, bits, block);
+  if (rc) {
+    return rc;
+  }
+  if (op != OP_COMPRESS && op != OP_DECOMPRESS) {
+    return set_error(FRSZ2_EARG, "op must be 0 (compress) or 1 (decompress)");
+  }
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const std::uint64_t value_bytes = n * (dtype == FRSZ2_FLOAT32 ? 4u : 8u);
+  if ((op == OP_COMPRESS ? in_bytes : out_bytes) < value_bytes) {
+    return set_error(FRSZ2_ECAPACITY, "value buffer holds fewer than n values");
+  }
+  auto launch = [&]() {
+    return op == OP_COMPRESS ? compress_device(dtype, bits, block, d_in, n, d_out, out_bytes, s)
+                             : decompress_device(dtype, bits, block, d_in, in_bytes, n, d_out, s);
+  };
+  // the argument checks run once, before any variant is timed
+  rc = launch();
+  if (rc) {
+    return rc;
+  }
+  reps = reps > 0 ? reps : 5;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FRSZ2_HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+  const hipError_t ee = hipEventCreate(&e1);
+  if (ee != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_status(ee, "hipEventCreate");
+  }
+  const variant before = get_variant(op, dtype, bits);
+  variant best = before;
+  float best_ms = -1.0f;
+  const int U = storage_bits(bits);
+  for (int vpt : { 2, 4, 8, 16 }) {
+    if (vpt == 2 && !(dtype == FRSZ2_FLOAT64 && U >= 16)) {
+      continue;
+    }
+    for (int stage = 0; stage <= (op == OP_DECOMPRESS ? 1 : 0); ++stage) {
+      for (int linear = 0; linear <= 1; ++linear) {
+        for (int small = 0; small <= 1; ++small) {
+          for (int nt = small; nt <= 1; ++nt) {
+            set_variant(op, dtype, bits, vpt, nt, stage, linear, small);
+            float ms = 0.0f;
+            hipError_t e = hipSuccess;
+            rc = launch(); // first launch of a variant loads its code object
+            if (!rc) {
+              e = hipEventRecord(e0, s);
+            }
+            for (int r = 0; !rc && e == hipSuccess && r < reps; ++r) {
+              rc = launch();
+            }
+            if (!rc && e == hipSuccess) {
+              e = hipEventRecord(e1, s);
+            }
+            if (!rc && e == hipSuccess) {
+              e = hipEventSynchronize(e1);
+            }
+            if (!rc && e == hipSuccess) {
+              e = hipEventElapsedTime(&ms, e0, e1);
+            }
+            if (rc || e != hipSuccess) {
+              set_variant(op, dtype, bits, before.vpt, before.nt, before.stage, before.linear, before.small);
+              (void)hipEventDestroy(e0);
+              (void)hipEventDestroy(e1);
+              return rc ? rc : hip_status(e, "variant timing");
+            }
+            if (best_ms < 0.0f || ms < best_ms) {
+              best_ms = ms;
+              best = variant{ vpt, nt, stage, linear, small };
+            }
+          }
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_variant(op, dtype, bits, best.vpt, best.nt, best.stage, best.linear, best.small);
+  if (values_per_thread) {
+    *values_per_thread = best.vpt;
+  }
+  if (flags) {
+    *flags = (best.nt ? FRSZ2_TUNE_NONTEMPORAL : 0) | (best.stage ? FRSZ2_TUNE_STAGED_STORES : 0) |
+             (best.linear ? FRSZ2_TUNE_LINEAR_ORDER : 0) | (best.small ? FRSZ2_TUNE_SMALL_WORKGROUPS : 0);
+  }
+  return FRSZ2_OK;
+}
+
+int
 frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags)
 {
   using namespace frsz2;

@@ -205,13 +205,13 @@ round_up(std::uint64_t x, std::uint64_t a)
   return (x + a - 1) / a * a;
 }
 
-// blocks per chunk: about CHUNK_BYTES of values, a multiple of 16 blocks
+// blocks per chunk: about CHUNK_BYTES of values, at least one block (a
+// chunk is staged as a compact sub-stream, so any block count keeps the
+// tile kernels' alignment)
 std::uint64_t
 chunk_blocks(std::uint64_t block, std::uint64_t vs)
 {
-  const std::uint64_t per = block * vs;
-  const std::uint64_t c = CHUNK_BYTES / per;
-  return std::max<std::uint64_t>(16, c & ~std::uint64_t{ 15 });
+  return std::max<std::uint64_t>(1, CHUNK_BYTES / (block * vs));
 }
 
 #define FRSZ2_PIPE_TRY(expr, what)                                                                            \

@@ -178,6 +178,29 @@ enum
 };
 int frsz2_tune_set(int op, int dtype, int bits, int values_per_thread, int flags);
 int frsz2_tune_get(int op, int dtype, int bits, int* values_per_thread, int* flags);
+/* Autotune the variant of (op, dtype, bits) on the caller's data: every
+ * variant (values per thread, non-temporal hint, staged stores, tile order,
+ * workgroup size) runs `reps` times (5 when reps <= 0) after one untimed
+ * launch, timed with events on `stream`, and the fastest is kept in the
+ * table (returned in *values_per_thread / *flags when not NULL).  op 0:
+ * d_in holds n values (in_bytes) and d_out receives the stream (out_bytes);
+ * op 1: d_in is a stream of n values (in_bytes) and d_out receives the
+ * values (out_bytes).  d_out is overwritten many times; every variant
+ * writes the same bytes.  Synchronous; calls on other threads meanwhile run
+ * with whatever variant is being timed (all are bit-exact). */
+int frsz2_tune_auto(int op,
+                    int dtype,
+                    int bits,
+                    uint64_t block,
+                    uint64_t n,
+                    const void* d_in,
+                    uint64_t in_bytes,
+                    void* d_out,
+                    uint64_t out_bytes,
+                    int reps,
+                    int* values_per_thread,
+                    int* flags,
+                    frsz2_stream_t stream);
 
 /* ---- batched: many arrays in one persistent, graph-captured launch ----
  * No counterpart in the reference: its kernels take one array per launch

@@ -193,3 +193,25 @@ def test_host_path_concurrent_callers(fz, port):
     assert rcs == [0, 0, 0, 0]
     for x, o in zip(xs, outs):
         assert np.array_equal(o, port.with_threads(8).compress(x, 16, 32))
+
+
+@pytest.mark.parametrize("dtype,bits,B", [(np.float32, 16, 32), (np.float64, 21, 5), (np.float32, 12, 1 << 22)])
+def test_host_path_tiny_and_empty(fz, port, dtype, bits, B):
+    """n = 0 (the stream is the zeroed pad only), n = 1 and a few values,
+    including a block size far above n (one short chunk)."""
+    from frsz_amd import _capi
+
+    code = 0 if dtype == np.float32 else 1
+    rng = np.random.default_rng(12)
+    for n in (0, 1, 3, 257):
+        x = values(rng, n, dtype)
+        want = port.compress(x, bits, B)
+        s = np.full(want.size, 0x99, np.uint8)
+        _capi.check(_capi.lib.frsz2_compress_host(code, bits, B, x.ctypes.data if n else None, n, s.ctypes.data,
+                                                  s.size))
+        assert np.array_equal(s, want), n
+        y = np.full(n, -3, dtype)
+        _capi.check(_capi.lib.frsz2_decompress_host(code, bits, B, s.ctypes.data, s.size, n,
+                                                    y.ctypes.data if n else None))
+        assert np.array_equal(ubits(y), ubits(port.decompress(want, n, dtype, bits, B))), n
+
