@@ -405,7 +405,13 @@ def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499):
     torch.cuda.empty_cache()
     return {"requests": nreq, "rate_offered": rate, "seconds": seconds, "hbm_budget_gib": budget_gib,
             "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "p999_ms": pct(lat, 99.9), "max_ms": pct(lat, 100),
-            "per_op": per, "throughput_req_s": round(nreq / makespan, 1),
+            "per_op": per,
+            # median of each stage: queued -> admitted (output carved from the
+            # pool), admitted -> launched, launched -> completion seen
+            "stages_p50_ms": {"admit": pct((res["t_admit"] - res["t_arrival"]) * 1e3, 50),
+                              "launch": pct((res["t_launch"] - res["t_admit"]) * 1e3, 50),
+                              "run": pct((res["t_done"] - res["t_launch"]) * 1e3, 50)},
+            "throughput_req_s": round(nreq / makespan, 1),
             "payload_gbs": round(payload / makespan / 1e9, 2), "all_ok": ok,
             "bit_exact_sample": {"checked": checked, "same": bool(same)},
             "launches": stats["launches"], "mean_requests_per_launch":
