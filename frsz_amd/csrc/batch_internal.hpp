@@ -50,9 +50,18 @@ int validate_job(const frsz2_job& j, int idx);
 std::uint64_t build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vector<bjob>& desc,
                                 std::vector<char>& tiled);
 
+// Descriptors a persistent launch can carry in its kernel arguments.
+constexpr int INLINE_JOBS = 8;
+
+// True when launch_persistent passes the descriptors in the kernel arguments
+// (a grid of one workgroup per tile, at most INLINE_JOBS descriptors, not
+// tile shape 3): d_desc is then not read and need not hold them.
+bool descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tiles);
+
 // Counter reset (when claiming) + persistent kernel over ndesc descriptors in
-// device memory (d_desc; h_desc is the host copy), on stream s.  Tile shape 3
-// launches one grouped kernel per class instead.
+// device memory (d_desc; h_desc is the host copy, which is what the kernel
+// reads when descriptors_inline holds), on stream s.  Tile shape 3 launches
+// one grouped kernel per class instead.
 hipError_t launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
                              unsigned long long* d_counter, const batch_shape& shape, hipStream_t s);
 

@@ -281,12 +281,12 @@ find_job(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, int hint)
   return lo;
 }
 
+// The persistent loop: tiles claimed `chunk` at a time from *counter, or
+// (chunk == 0) taken round-robin by workgroup index.
 template<int SHAPE, int OPS>
-__global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
-                                              int njobs,
-                                              std::uint64_t total_tiles,
-                                              unsigned long long* counter,
-                                              int chunk)
+__device__ __forceinline__ void
+batch_loop(const bjob* __restrict__ jobs, int njobs, std::uint64_t total_tiles, unsigned long long* counter,
+           int chunk)
 {
   __shared__ std::uint32_t lds[batch_lds_dwords<SHAPE>()];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
@@ -321,6 +321,29 @@ __global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
       run_tile<SHAPE, OPS>(J, g - J.tile0, lds, wred);
     }
   }
+}
+
+template<int SHAPE, int OPS>
+__global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
+                                              int njobs,
+                                              std::uint64_t total_tiles,
+                                              unsigned long long* counter,
+                                              int chunk)
+{
+  batch_loop<SHAPE, OPS>(jobs, njobs, total_tiles, counter, chunk);
+}
+
+// Up to INLINE_JOBS descriptors passed in the kernel arguments: a launch of
+// a few jobs needs no host-to-device copy of its descriptors before it.
+struct job_pack
+{
+  bjob j[INLINE_JOBS];
+};
+
+template<int SHAPE, int OPS>
+__global__ __launch_bounds__(WG) void k_batch_inline(job_pack pack, int njobs, std::uint64_t total_tiles)
+{
+  batch_loop<SHAPE, OPS>(pack.j, njobs, total_tiles, nullptr, 0);
 }
 
 __global__ void
@@ -529,6 +552,20 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
   return tiles;
 }
 
+static std::uint64_t
+persistent_grid(const batch_shape& shape, std::uint64_t total_tiles)
+{
+  const std::uint64_t grid = static_cast<std::uint64_t>(shape.workgroups);
+  return grid < total_tiles ? grid : total_tiles;
+}
+
+bool
+descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tiles)
+{
+  return shape.tile_shape != 3 && ndesc > 0 && ndesc <= INLINE_JOBS && total_tiles > 0 &&
+         persistent_grid(shape, total_tiles) == total_tiles;
+}
+
 hipError_t
 launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
                   unsigned long long* d_counter, const batch_shape& shape, hipStream_t s)
@@ -539,19 +576,27 @@ launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64
   if (shape.tile_shape == 3) {
     return launch_grouped(d_desc, h_desc, ndesc, total_tiles, s);
   }
-  const int chunk = shape.tiles_per_claim;
-  std::uint64_t grid = static_cast<std::uint64_t>(shape.workgroups);
-  if (grid > total_tiles) {
-    grid = total_tiles;
-  }
+  const std::uint64_t grid = persistent_grid(shape, total_tiles);
+  // a grid that covers every tile takes one tile per workgroup: no claims,
+  // so no counter to reset
+  const int chunk = grid == total_tiles ? 0 : shape.tiles_per_claim;
+  const bool inl = h_desc != nullptr && descriptors_inline(shape, ndesc, total_tiles);
   if (chunk > 0) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
   }
   const dim3 gd(static_cast<unsigned>(grid));
+  job_pack pack{};
+  if (inl) {
+    std::copy(h_desc, h_desc + ndesc, pack.j);
+  }
   // one kernel instance per (tile shape, set of directions present)
   const int ops = shape.ops == 0 ? 7 : shape.ops == 1 ? 1 : (shape.ops & 1) != 0 ? 7 : 6;
 #define FRSZ2_KB(S, O)                                                                                       \
-  hipLaunchKernelGGL((k_batch<S, O>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk)
+  if (inl) {                                                                                                 \
+    hipLaunchKernelGGL((k_batch_inline<S, O>), gd, dim3(WG), 0, s, pack, ndesc, total_tiles);                \
+  } else {                                                                                                   \
+    hipLaunchKernelGGL((k_batch<S, O>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);   \
+  }
   switch (shape.tile_shape * 8 + ops) {
     case 0 * 8 + 1: FRSZ2_KB(0, 1); break;
     case 0 * 8 + 6: FRSZ2_KB(0, 6); break;

@@ -361,16 +361,19 @@ launch_slot(frsz2_server* S, slot& sl)
   const std::uint64_t tiles = build_descriptors(sl.jobs.data(), sl.jobs.size(), S->cfg.tile_shape, sl.desc, sl.tiled);
   const int nd = static_cast<int>(sl.desc.size());
   if (nd > 0) {
-    std::copy(sl.desc.begin(), sl.desc.end(), sl.h_desc);
-    hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, nd * sizeof(bjob), hipMemcpyHostToDevice, sl.stream);
-    if (e != hipSuccess) {
-      return hip_status(e, "hipMemcpyAsync");
-    }
     int ops = 0;
     for (const frsz2_job& j : sl.jobs) {
       ops |= 1 << j.op;
     }
     const batch_shape shape{ S->cfg.tile_shape, S->cfg.workgroups, S->cfg.tiles_per_claim, ops };
+    hipError_t e = hipSuccess;
+    if (!descriptors_inline(shape, nd, tiles)) {
+      std::copy(sl.desc.begin(), sl.desc.end(), sl.h_desc);
+      e = hipMemcpyAsync(sl.d_desc, sl.h_desc, nd * sizeof(bjob), hipMemcpyHostToDevice, sl.stream);
+      if (e != hipSuccess) {
+        return hip_status(e, "hipMemcpyAsync");
+      }
+    }
     e = launch_persistent(sl.d_desc, sl.desc.data(), nd, tiles, sl.d_counter, shape, sl.stream);
     if (e != hipSuccess) {
       return hip_status(e, "batch launch");

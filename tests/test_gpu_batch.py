@@ -79,6 +79,36 @@ def test_batch_matches_plain(fz, cfg):
     dplan.close()
 
 
+@pytest.mark.parametrize("count,cfg", [(1, None), (3, {"use_graph": 0}), (8, {"use_graph": 1}),
+                                       (9, {"use_graph": 0}), (5, {"workgroups": 16, "tiles_per_claim": 2}),
+                                       (4, {"workgroups": 16, "tiles_per_claim": 0, "use_graph": 1})])
+def test_few_jobs(fz, count, cfg):
+    """Up to eight descriptors over a grid of one workgroup per tile travel in
+    the kernel arguments (no descriptor copy); nine, or a grid smaller than
+    the tile count, read them from device memory."""
+    fz_mod, batch = fz
+    specs = make_arrays(30 + count, count=count)
+    ref_s, ref_v = plain_outputs(fz_mod, specs)
+    outs = [torch.full_like(s, 0x77) for s in ref_s]
+    plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
+    vals = [torch.full_like(x, float("nan")) for x, _, _ in specs]
+    dplan = batch.BatchPlan([batch.decompress_job(s, v, b, B) for (x, b, B), s, v in zip(specs, ref_s, vals)], cfg)
+    for rep in range(2):
+        plan.launch()
+        dplan.launch()
+        torch.cuda.synchronize()
+        for i, (o, r) in enumerate(zip(outs, ref_s)):
+            assert torch.equal(o, r), f"rep {rep} job {i}"
+        for i, (v, r) in enumerate(zip(vals, ref_v)):
+            assert torch.equal(v.view(torch.uint8), r.view(torch.uint8)), f"rep {rep} decompress job {i}"
+        for o in outs:
+            o.fill_(0x33)
+        for v in vals:
+            v.fill_(float("nan"))
+    plan.close()
+    dplan.close()
+
+
 def test_graph_replays_with_eager_work_between(fz):
     fz_mod, batch = fz
     specs = make_arrays(2, count=30)
