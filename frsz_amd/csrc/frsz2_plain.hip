@@ -55,10 +55,15 @@ __global__ __launch_bounds__(WGS) void k_decompress_tile(const std::uint8_t* __r
                                                          std::uint32_t ntiles,
                                                          int linear)
 {
-  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
-  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
+  // one array, the payload image first: it then starts at LDS address 0 and
+  // the code extraction's addresses need no base (a multiple of 4 dwords, so
+  // the output stage after it is 16-byte aligned)
+  constexpr int LD = PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 0;
+  constexpr int SD = STAGE ? tile::stage_dwords<T, VPT, WGS>() : 0;
+  static_assert(LD % 4 == 0, "stage alignment");
+  __shared__ __attribute__((aligned(16))) std::uint32_t smem[LD + SD > 0 ? LD + SD : 4];
   const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
-  tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE, WGS>(in, out, p, t, lds, stage);
+  tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE, WGS>(in, out, p, t, smem, smem + LD);
 }
 
 // One thread per exponent block, any B (encode_block: each payload word is

@@ -403,19 +403,20 @@ or_chunk(std::uint32_t* lds, std::uint32_t pos0, const UW (&code)[VC], int bits)
   }
 }
 
+// The U bits at bit offset pos of the LDS payload image: a code in the low
+// `bits` bits, with the bits above it left in (the decodes ignore them).
 template<int U>
 __device__ __forceinline__ std::uint64_t
-extract_code(const std::uint32_t* lds, std::uint32_t pos, int bits)
+extract_code(const std::uint32_t* lds, std::uint32_t pos)
 {
   const std::uint32_t q = pos >> 5;
   const std::uint32_t o = pos & 31u;
   const std::uint32_t lo = __builtin_amdgcn_alignbit(lds[q + 1], lds[q], o);
   if constexpr (U <= 32) {
-    return bits == 32 ? lo : (lo & ((1u << bits) - 1u));
+    return lo;
   } else {
     const std::uint32_t hi = __builtin_amdgcn_alignbit(lds[q + 2], lds[q + 1], o);
-    const std::uint64_t c = static_cast<std::uint64_t>(lo) | (static_cast<std::uint64_t>(hi) << 32);
-    return bits == 64 ? c : (c & ((std::uint64_t{ 1 } << bits) - 1));
+    return static_cast<std::uint64_t>(lo) | (static_cast<std::uint64_t>(hi) << 32);
   }
 }
 
@@ -659,11 +660,12 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
   }
 }
 
-// Values of a chunk of VC codes.  `uniform`: the chunk lies in one exponent
-// block (E[0] holds for all of it, so the per-block terms are computed once);
-// `narrow`: bits <= SB + 1, where decode_value_short applies; both together
-// take the short path.  Both flags are the same for every thread of the
-// launch.
+// Values of a chunk of VC codes (bits of a code above `bits` are ignored).
+// `uniform`: the chunk lies in one exponent block (E[0] holds for all of it,
+// so the per-block terms are computed once); `narrow`: bits <= SB + 1.  Both
+// flags are the same for every thread of the launch; together with
+// E + bias >= 0 (true for every block a compress writes; a stream from
+// elsewhere may hold any exponent) they take decode_value_short.
 template<typename T, int VC>
 __device__ __forceinline__ void
 decode_chunk(const typename fmt<T>::uw (&code)[VC],
@@ -673,7 +675,7 @@ decode_chunk(const typename fmt<T>::uw (&code)[VC],
              int bits,
              typename fmt<T>::uw (&y)[VC])
 {
-  if (narrow && uniform) {
+  if (narrow && uniform && E[0] + fmt<T>::BIAS >= 0) {
     const int e = E[0];
 #pragma unroll
     for (int v = 0; v < VC; ++v) {
@@ -720,6 +722,118 @@ stage_dwords()
   return WGS * VPT * static_cast<int>(sizeof(T)) / 4;
 }
 
+// Decompress one tile of packed codes (bits < U), strided: thread t decodes
+// tile values k*WGS + t (k < VPT).  The bit offset of tile value i in the
+// tile's payload is f(i) = (i >> lb)*CB*8 + (i & (B-1))*bits, and for
+// power-of-two B, f(k*WGS + t) = f(k*WGS) + f(t) (B >= WGS: k*WGS mod B + t
+// < B; B < WGS: k*WGS is a multiple of B), so a value's offset is one add of
+// a workgroup-uniform term.  Each wave instruction reads consecutive dwords
+// of the LDS image (no bank conflicts, whatever bits is) and stores 64
+// consecutive values, so no output staging is needed.  A value's exponent
+// is loaded for it (lanes of one block read one address).
+template<typename T, int U, int VPT, bool NT, int WGS>
+__device__ __forceinline__ void
+decompress_tile_strided(const std::uint8_t* __restrict__ in,
+                        typename fmt<T>::uw* __restrict__ out,
+                        const tplan& p,
+                        std::uint64_t tile,
+                        std::uint32_t* lds)
+{
+  using uw = typename fmt<T>::uw;
+  constexpr int TILE = WGS * VPT;
+  const std::uint32_t t = threadIdx.x;
+  const std::uint64_t base = tile * TILE;
+  const std::uint32_t bm = (1u << p.lb) - 1u;
+  const std::uint32_t cb8 = p.CB * 8u;
+  const std::uint32_t bits = static_cast<std::uint32_t>(p.bits);
+
+  const std::uint64_t k0 = base >> p.lb;
+  const bool full = base + TILE <= p.n;
+
+  // exponents first: their loads are in flight with the payload's.  Value
+  // k*WGS + t is in tile block (k*WGS >> lb) + (t >> lb) (power-of-two B), a
+  // uniform part and a lane part.
+  using et = typename signed_word<U>::type;
+  const std::uint32_t tl = t >> p.lb;
+  const et* e0 = reinterpret_cast<const et*>(in) + k0;
+  int E[VPT];
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const et* ek = e0 + (static_cast<std::uint32_t>(k * WGS) >> p.lb);
+      E[k] = ek[tl];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const std::uint32_t kw = static_cast<std::uint32_t>(k * WGS);
+      E[k] = base + kw + t < p.n ? static_cast<int>(e0[(kw >> p.lb) + tl]) : 0;
+    }
+  }
+
+  const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
+  const std::uint64_t left = p.nb - k0;
+  const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
+  const std::uint32_t nbytes = blocks * p.CB;
+  const std::uint8_t* src = in + p.R + k0 * p.CB;
+  const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
+  u32x4* l4 = reinterpret_cast<u32x4*>(lds);
+  for (std::uint32_t c = t; c < nbytes / 16; c += WGS) {
+    l4[c] = ld<NT>(s4 + c);
+  }
+  const std::uint32_t tail = nbytes & 15u;
+  if (t < tail) {
+    const std::uint32_t o = nbytes - tail + t;
+    reinterpret_cast<std::uint8_t*>(lds)[o] = src[o];
+  }
+  __syncthreads();
+
+  const std::uint32_t P = (t >> p.lb) * cb8 + (t & bm) * bits;
+  uw code[VPT];
+#pragma unroll
+  for (int k = 0; k < VPT; ++k) {
+    const std::uint32_t kw = static_cast<std::uint32_t>(k * WGS);
+    // uniform; kept whole so the per-value work is the one add
+    const std::uint32_t sk = __builtin_amdgcn_readfirstlane((kw >> p.lb) * cb8 + (kw & bm) * bits);
+    code[k] = static_cast<uw>(extract_code<U>(lds, P + sk));
+  }
+  // decode_value_short needs bits <= SB + 1 and E + bias >= 0 (every block a
+  // compress writes); the check is per thread, over all of its values
+  int emin = E[0];
+#pragma unroll
+  for (int k = 1; k < VPT; ++k) {
+    emin = E[k] < emin ? E[k] : emin;
+  }
+  uw y[VPT];
+  if (p.bits <= fmt<T>::SB + 1 && emin + fmt<T>::BIAS >= 0) {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      y[k] = decode_value_short<T>(code[k], E[k], p.bits);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      y[k] = decode_value<T>(code[k], E[k], p.bits);
+    }
+  }
+  if (full) {
+    uw* ob = out + base;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      st<NT>(y[k], ob + static_cast<std::uint32_t>(k * WGS) + t);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const std::uint64_t i = base + static_cast<std::uint32_t>(k * WGS) + t;
+      if (i < p.n) {
+        out[i] = y[k];
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int WGS = WG>
 __device__ __forceinline__ void
 decompress_tile(const std::uint8_t* __restrict__ in,
@@ -729,6 +843,10 @@ decompress_tile(const std::uint8_t* __restrict__ in,
                 std::uint32_t* lds,
                 std::uint32_t* stage)
 {
+  if constexpr (PACKED && !STAGE) {
+    decompress_tile_strided<T, U, VPT, NT, WGS>(in, out, p, tile, lds);
+    return;
+  }
   using uw = typename fmt<T>::uw;
   constexpr int TILE = WGS * VPT;
   // values per chunk.  Staged: 16 bytes of codes (or all of a thread's
@@ -813,6 +931,15 @@ decompress_tile(const std::uint8_t* __restrict__ in,
           code[c][v] = (i0 + v < limit) ? static_cast<uw>(reinterpret_cast<const wt*>(pay)[i0 + v]) : uw{ 0 };
         }
       }
+    } else if (uniform) {
+      // one block: the chunk's codes are consecutive in the bitstream
+      std::uint32_t pos = (local0 >> p.lb) * p.CB * 8u +
+                          (local0 & static_cast<std::uint32_t>(B - 1)) * static_cast<std::uint32_t>(p.bits);
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        code[c][v] = static_cast<uw>(extract_code<U>(lds, pos));
+        pos += static_cast<std::uint32_t>(p.bits);
+      }
     } else {
 #pragma unroll
       for (int v = 0; v < VC; ++v) {
@@ -820,7 +947,7 @@ decompress_tile(const std::uint8_t* __restrict__ in,
         const std::uint32_t kl = local >> p.lb;
         const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
         const std::uint32_t pos = kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits);
-        code[c][v] = static_cast<uw>(extract_code<U>(lds, pos, p.bits));
+        code[c][v] = static_cast<uw>(extract_code<U>(lds, pos));
       }
     }
   }

@@ -1,13 +1,14 @@
 // Default kernel variants of the tiled kernels, measured on MI355X (1 GiB
 // float64 field; 10^8-value float32 and 5*10^7-value float64 arrays of the
 // edge-case generator), shared by the tuning table of the plain kernels and
-// the grouped batch kernels (which always run 256-thread workgroups).
+// the grouped batch kernels.
 #pragma once
 
 namespace frsz2 {
 
-// encoded variant: values per thread, staged decompress stores, non-temporal
-// hint, linear (workgroup-order) tiles, 64-thread workgroups (plain kernels)
+// encoded variant: values per thread, staged decompress (packed codes: the
+// chunked decode with LDS-staged stores; off: the strided decode), the
+// non-temporal hint, linear (workgroup-order) tiles, 64-thread workgroups
 constexpr int
 venc(int vpt, int stage, int nt, int linear = 0, int small = 0)
 {
@@ -38,12 +39,12 @@ default_variant(int op, int dtype, int ui, int packed)
   }
   if (dtype == 0) {
     if (packed) {
-      return ui == 2 ? venc(8, 1, 1, 1, 1) : venc(16, 1, 1, 1, 1);
+      return ui == 0 ? venc(16, 1, 1, 1) : ui == 1 ? venc(8, 0, 1, 1) : venc(16, 0, 1, 1, 1);
     }
     return ui == 1 ? venc(16, 1, 1, 1, 1) : venc(16, 1, 1, 1);
   }
   if (packed) {
-    return ui == 3 ? venc(4, 1, 1, 1, 1) : venc(8, 1, 1, 1);
+    return ui <= 1 ? venc(8, 1, 1, 1) : ui == 2 ? venc(4, 0, 1, 1) : venc(2, 0, 1, 1);
   }
   return venc(8, 1, 1, 1);
 }

@@ -111,6 +111,26 @@ def test_short_decode_float64_equals_generic(dm):
             assert np.array_equal(a, b), (bits, E)
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_decodes_ignore_bits_above_the_code(dm, wide):
+    """The packed-code extraction hands the decodes a whole word whose bits
+    above `bits` belong to the next codes: both decodes must ignore them."""
+    rng = np.random.default_rng(6)
+    ut, W, SB, bias = (np.uint64, 64, 52, 1023) if wide else (np.uint32, 32, 23, 127)
+    name = "f64" if wide else "f32"
+    for bits in range(4, SB + 2):
+        codes = rng.integers(0, 1 << bits, 2048, dtype=ut)
+        codes[:8] = np.array([0, 1, (1 << (bits - 1)), (1 << bits) - 1, 2, 3, 1 << (bits - 2), 5], ut)
+        junk = (rng.integers(0, np.iinfo(ut).max, 2048, dtype=ut, endpoint=True) >> ut(bits)) << ut(bits)
+        for E in ([-bias, -bias + 1, -1, 0, 1, 30, bias] + list(rng.integers(-bias, bias + 1, 4))):
+            E = int(E)
+            if wide and bits <= 8 and not -128 <= E <= 127:
+                continue
+            want = apply(dm, "dm_decode_" + name, codes, E, bits)
+            assert np.array_equal(apply(dm, "dm_decode_" + name, codes | junk, E, bits), want), (bits, E)
+            assert np.array_equal(apply(dm, "dm_decode_short_" + name, codes | junk, E, bits), want), (bits, E)
+
+
 def test_short_encode_float32_equals_generic(dm):
     """encode_value_short (bits <= 24) against encode_value on every biased
     exponent (NaN patterns included) for every block exponent E >= exponent(x)."""

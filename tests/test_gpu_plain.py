@@ -204,3 +204,45 @@ def test_host_memory_entry_points(fz, port):
                                             g.ctypes.data))
     assert np.array_equal(ubits(g), ubits(y[idx.astype(np.int64)]))
     del ctypes
+
+
+ARBITRARY = [(np.float32, 12, 32), (np.float32, 12, 4), (np.float32, 8, 32), (np.float32, 16, 16),
+             (np.float32, 21, 64), (np.float32, 24, 32), (np.float32, 32, 32), (np.float64, 21, 32),
+             (np.float64, 40, 16), (np.float64, 8, 32), (np.float64, 53, 8), (np.float64, 60, 32),
+             (np.float64, 12, 1024)]
+
+
+@pytest.mark.parametrize("dtype,bits,B", ARBITRARY)
+def test_decompress_arbitrary_streams(fz, port, dtype, bits, B):
+    """Streams that no compress writes: random payload bytes and block
+    exponents over the range where the reference's decode is defined (its
+    shift counts stay below the width: E + bias >= -24 for float32, >= -53
+    for float64, up to 40000), which includes E + bias < 0, a range no
+    compress writes.  Decompress, block runs and gather equal the oracle."""
+    from oracle.oracle import compressed_bytes, storage_bits
+
+    print(f"arbitrary {np.dtype(dtype).name} b{bits} B{B}", flush=True)
+    rng = np.random.default_rng(bits * 7 + B)
+    n = 50_003
+    U = storage_bits(bits)
+    nb = -(-n // B)
+    s = rng.integers(0, 256, compressed_bytes(bits, B, n), dtype=np.uint8)
+    et = {8: np.int8, 16: np.int16, 32: np.int32, 64: np.int64}[U]
+    info = np.iinfo(et)
+    lo = max(int(info.min), -151 if dtype == np.float32 else -1076)
+    hi = min(info.max, 40_000)
+    E = rng.integers(lo, hi, nb, endpoint=True).astype(et)
+    E[: min(nb, 4)] = np.array([lo, hi, -1, 0], dtype=et)[: min(nb, 4)]
+    s[: nb * U // 8] = E.view(np.uint8)
+    want = port.decompress(s, n, dtype, bits, B)
+    sd = torch.from_numpy(s).cuda()
+    y = fz.decompress(sd, n, tdt(dtype), bits, B).cpu().numpy()
+    assert np.array_equal(ubits(y), ubits(want))
+    yb = torch.empty(n, dtype=tdt(dtype), device="cuda")
+    cut = (nb // 3) * B
+    fz.decompress_blocks(sd, yb[:cut], n, 0, bits, B)
+    fz.decompress_blocks(sd, yb[cut:], n, nb // 3, bits, B)
+    assert np.array_equal(ubits(yb.cpu().numpy()), ubits(want))
+    idx = rng.integers(0, n, 4096).astype(np.int64)
+    g = fz.gather(sd, n, tdt(dtype), bits, B, torch.from_numpy(idx).cuda()).cpu().numpy()
+    assert np.array_equal(ubits(g), ubits(want[idx]))
