@@ -358,27 +358,39 @@ or_code(std::uint32_t* lds, std::uint32_t pos, std::uint64_t code, int bits)
 // OR the VC codes of one chunk into the LDS bit image when the chunk lies in
 // one exponent block: its codes are then consecutive in the bitstream, from
 // bit pos0 on.  They are assembled into one 128-bit value (VC * bits <= 128;
-// the offsets v * bits are the same for every lane, so the placement
-// branches are uniform), shifted by pos0 mod 32 and OR-ed in as the 1..5
-// dwords they touch: one LDS atomic per touched dword instead of one or two
-// per code.
+// codes are below 2^bits, and bits < U, so pairs of float32 codes fit one
+// 64-bit word), shifted by pos0 mod 32 and OR-ed in as the 1..5 dwords they
+// touch: one LDS atomic per touched dword instead of one or two per code.
 template<int VC, typename UW>
 __device__ __forceinline__ void
 or_chunk(std::uint32_t* lds, std::uint32_t pos0, const UW (&code)[VC], int bits)
 {
   std::uint64_t lo = 0;
   std::uint64_t hi = 0;
+  if constexpr (VC == 4) {
+    // float32 chunk, packed codes (bits <= 31): two codes fit one 64-bit word
+    const std::uint64_t p01 = static_cast<std::uint64_t>(code[0]) | (static_cast<std::uint64_t>(code[1]) << bits);
+    const std::uint64_t p23 = static_cast<std::uint64_t>(code[2]) | (static_cast<std::uint64_t>(code[3]) << bits);
+    const int s2 = 2 * bits;
+    lo = p01 | (p23 << s2);
+    hi = p23 >> (64 - s2);
+  } else if constexpr (VC == 2) {
+    // float64 chunk, packed codes (bits <= 63)
+    lo = static_cast<std::uint64_t>(code[0]) | (static_cast<std::uint64_t>(code[1]) << bits);
+    hi = static_cast<std::uint64_t>(code[1]) >> (64 - bits);
+  } else {
 #pragma unroll
-  for (int v = 0; v < VC; ++v) {
-    const std::uint64_t c = static_cast<std::uint64_t>(code[v]);
-    const int sh = v * bits;
-    if (sh == 0) {
-      lo |= c;
-    } else if (sh < 64) {
-      lo |= c << sh;
-      hi |= c >> (64 - sh);
-    } else {
-      hi |= c << (sh - 64);
+    for (int v = 0; v < VC; ++v) {
+      const std::uint64_t c = static_cast<std::uint64_t>(code[v]);
+      const int sh = v * bits;
+      if (sh == 0) {
+        lo |= c;
+      } else if (sh < 64) {
+        lo |= c << sh;
+        hi |= c >> (64 - sh);
+      } else {
+        hi |= c << (sh - 64);
+      }
     }
   }
   const std::uint32_t q = pos0 >> 5;
@@ -619,13 +631,19 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
     }
     __syncthreads();
     if (B >= VC) {
-      // each chunk lies in one block
+      // each chunk lies in one block.  The bit offset of tile value i is
+      // f(i) = (i >> lb)*CB*8 + (i & (B-1))*bits, and for power-of-two B,
+      // f(c*WGS*VC + t*VC) = f(c*WGS*VC) + f(t*VC): a uniform term per chunk
+      const std::uint32_t cb8 = p.CB * 8u;
+      const std::uint32_t bm = static_cast<std::uint32_t>(B - 1);
+      const std::uint32_t bits = static_cast<std::uint32_t>(p.bits);
+      const std::uint32_t tv = static_cast<std::uint32_t>(t * VC);
+      const std::uint32_t P = (tv >> p.lb) * cb8 + (tv & bm) * bits;
 #pragma unroll
       for (int c = 0; c < NCH; ++c) {
-        const std::uint32_t local = static_cast<std::uint32_t>((c * WGS + t) * VC);
-        const std::uint32_t kl = local >> p.lb;
-        const std::uint32_t j = local & static_cast<std::uint32_t>(B - 1);
-        or_chunk<VC>(lds, kl * p.CB * 8u + j * static_cast<std::uint32_t>(p.bits), code[c], p.bits);
+        const std::uint32_t cw = static_cast<std::uint32_t>(c * WGS * VC);
+        const std::uint32_t sc = __builtin_amdgcn_readfirstlane((cw >> p.lb) * cb8 + (cw & bm) * bits);
+        or_chunk<VC>(lds, P + sc, code[c], p.bits);
       }
     } else {
 #pragma unroll
