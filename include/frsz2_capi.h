@@ -162,7 +162,9 @@ int frsz2_decompress_blocks(int dtype,
  * and is raised to the next size that holds a whole exponent block;
  * flags: FRSZ2_TUNE_NONTEMPORAL puts the non-temporal hint on the streaming
  * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) writes each tile's output
- * through LDS with whole-cache-line stores, FRSZ2_TUNE_LINEAR_ORDER takes
+ * through LDS with whole-cache-line stores (for bits < U it selects the
+ * decode of consecutive values per thread over the strided decode, which
+ * stores straight from registers), FRSZ2_TUNE_LINEAR_ORDER takes
  * tiles in workgroup order (default: one contiguous range of tiles per
  * XCD), FRSZ2_TUNE_SMALL_WORKGROUPS launches 64-thread workgroups (one wave,
  * tiles of 64 x values_per_thread; implies the non-temporal hint) instead of
