@@ -155,3 +155,46 @@ def test_batch_rejects_bad_jobs(fz):
     with pytest.raises(fz_mod.Frsz2Error) as e:
         batch.BatchPlan([batch.compress_job(x, out, 33, 32)])
     assert e.value.code == 2 and "Unsupported number of bits: 33" in str(e.value)
+
+
+@pytest.mark.parametrize("cfg", [None, {"tile_shape": 0, "use_graph": 0}, {"tile_shape": 2, "use_graph": 0},
+                                 {"tile_shape": 3, "use_graph": 1}], ids=["default", "narrow", "wide", "grouped"])
+def test_decompress_and_gather_of_arbitrary_streams(fz, cfg):
+    """Streams no compress writes (random payload and block exponents over
+    the range where the reference decode is defined, E + bias < 0 included):
+    every batched decompress and gather equals the plain call, for each tile
+    shape."""
+    fz_mod, batch = fz
+    from oracle.oracle import compressed_bytes, storage_bits
+
+    rng = np.random.default_rng(17)
+    streams, outs, gouts, idxs, specs = [], [], [], [], []
+    for i, (dt, bits, B) in enumerate([(torch.float32, 12, 32), (torch.float32, 21, 16), (torch.float32, 16, 64),
+                                       (torch.float32, 7, 8), (torch.float64, 21, 32), (torch.float64, 40, 128),
+                                       (torch.float64, 32, 32), (torch.float64, 9, 4), (torch.float32, 32, 1024)]):
+        n = int(rng.integers(1, 60_000))
+        U = storage_bits(bits)
+        nb = -(-n // B)
+        s = rng.integers(0, 256, compressed_bytes(bits, B, n), dtype=np.uint8)
+        et = {8: np.int8, 16: np.int16, 32: np.int32, 64: np.int64}[U]
+        lo = max(int(np.iinfo(et).min), -151 if dt == torch.float32 else -1076)
+        hi = min(int(np.iinfo(et).max), 40_000)
+        s[: nb * U // 8] = rng.integers(lo, hi, nb, endpoint=True).astype(et).view(np.uint8)
+        sd = torch.from_numpy(s).cuda()
+        streams.append(sd)
+        outs.append(torch.full((n,), float("nan"), dtype=dt, device="cuda"))
+        idx = torch.from_numpy(rng.integers(0, n, 777).astype(np.int64)).cuda()
+        idxs.append(idx)
+        gouts.append(torch.full((777,), float("nan"), dtype=dt, device="cuda"))
+        specs.append((n, dt, bits, B))
+    jobs = [batch.decompress_job(s, o, b, B) for s, o, (n, dt, b, B) in zip(streams, outs, specs)]
+    jobs += [batch.gather_job(s, n, g, ix, b, B) for s, g, ix, (n, dt, b, B) in zip(streams, gouts, idxs, specs)]
+    plan = batch.BatchPlan(jobs, cfg)
+    plan.launch()
+    torch.cuda.synchronize()
+    for i, (s, o, g, ix, (n, dt, b, B)) in enumerate(zip(streams, outs, gouts, idxs, specs)):
+        want = fz_mod.decompress(s, n, dt, b, B)
+        assert torch.equal(o.view(torch.uint8), want.view(torch.uint8)), f"decompress job {i}: {dt} b{b} B{B} n{n}"
+        wg = fz_mod.gather(s, n, dt, b, B, ix)
+        assert torch.equal(g.view(torch.uint8), wg.view(torch.uint8)), f"gather job {i}"
+    plan.close()
