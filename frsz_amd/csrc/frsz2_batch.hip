@@ -355,13 +355,18 @@ k_reset_counter(unsigned long long* counter)
 // Grouped mode: one workgroup per tile of the jobs of one class (jobs[0..njobs)
 // are that class's descriptors, tiles tile_base + blockIdx.x).
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
-__global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base)
+__global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs,
+                                                 int njobs,
+                                                 std::uint64_t tile_base,
+                                                 const std::uint32_t* __restrict__ jtab)
 {
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
   __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
   __shared__ std::uint32_t wred[WGS / tile::WAVE];
   const std::uint64_t g = tile_base + blockIdx.x;
-  const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, 0));
+  // the plan's tile table gives the job in one load; without one, a binary
+  // search over the class's descriptors
+  const int j = jtab ? static_cast<int>(jtab[g]) : __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, 0));
   const bjob J = jobs[j];
   const std::uint64_t t = g - J.tile0;
   if constexpr (OP == 2) {
@@ -385,7 +390,7 @@ __global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs, 
 
 template<int CLS>
 void
-launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, hipStream_t s)
+launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, const std::uint32_t* jtab, hipStream_t s)
 {
   using T = std::conditional_t<cls_dtype(CLS) == 0, float, double>;
   constexpr int U = 8 << cls_ui(CLS);
@@ -397,7 +402,7 @@ launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, hipStrea
   constexpr bool STAGE = OP == 1 && venc_stage(V) != 0;
   constexpr int WGS = class_wgs(CLS, 3);
   hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)), dim3(WGS),
-                     0, s, d, nd, t0);
+                     0, s, d, nd, t0, jtab);
 }
 
 int
@@ -422,7 +427,8 @@ aligned16(const void* p)
 // grouped mode: one k_grouped launch per run of same-class descriptors
 // (descriptors are built in class order)
 hipError_t
-launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, hipStream_t s)
+launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, const std::uint32_t* jtab,
+               hipStream_t s)
 {
   int d0 = 0;
   while (d0 < nd) {
@@ -436,7 +442,7 @@ launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t tot
     if (t1 > t0) {
 #define FRSZ2_G(K)                                                                                           \
   case K:                                                                                                    \
-    launch_class<K>(d_desc + d0, d1 - d0, t0, t1 - t0, s);                                                   \
+    launch_class<K>(d_desc + d0, d1 - d0, t0, t1 - t0, jtab, s);                                             \
     break
       switch (cls) {
         FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
@@ -552,6 +558,24 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
   return tiles;
 }
 
+std::vector<std::uint32_t>
+tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles)
+{
+  std::vector<std::uint32_t> tab(total_tiles, 0u);
+  const std::size_t nd = desc.size();
+  std::size_t d0 = 0; // first descriptor of the current class
+  for (std::size_t d = 0; d < nd; ++d) {
+    if (desc[d].cls != desc[d0].cls) {
+      d0 = d;
+    }
+    const std::uint64_t t1 = d + 1 < nd ? desc[d + 1].tile0 : total_tiles;
+    for (std::uint64_t t = desc[d].tile0; t < t1; ++t) {
+      tab[t] = static_cast<std::uint32_t>(d - d0);
+    }
+  }
+  return tab;
+}
+
 static std::uint64_t
 persistent_grid(const batch_shape& shape, std::uint64_t total_tiles)
 {
@@ -568,13 +592,13 @@ descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tile
 
 hipError_t
 launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
-                  unsigned long long* d_counter, const batch_shape& shape, hipStream_t s)
+                  unsigned long long* d_counter, const batch_shape& shape, hipStream_t s, const std::uint32_t* d_jtab)
 {
   if (ndesc == 0 || total_tiles == 0) {
     return hipSuccess;
   }
   if (shape.tile_shape == 3) {
-    return launch_grouped(d_desc, h_desc, ndesc, total_tiles, s);
+    return launch_grouped(d_desc, h_desc, ndesc, total_tiles, d_jtab, s);
   }
   const std::uint64_t grid = persistent_grid(shape, total_tiles);
   // a grid that covers every tile takes one tile per workgroup: no claims,
@@ -645,6 +669,7 @@ struct frsz2_batch_plan
   std::vector<char> tiled;            // job index -> 1 if it takes the persistent kernel
   std::vector<frsz2::bjob> host_desc; // descriptors for cfg.tile_shape
   frsz2::bjob* d_desc = nullptr;
+  std::uint32_t* d_jtab = nullptr; // tile shape 3: tile -> descriptor index within its class
   unsigned long long* d_counter = nullptr;
   int ndesc = 0;
   std::uint64_t total_tiles = 0;
@@ -668,6 +693,10 @@ upload_descriptors(frsz2_batch_plan* P)
     (void)hipFree(P->d_desc);
     P->d_desc = nullptr;
   }
+  if (P->d_jtab) {
+    (void)hipFree(P->d_jtab);
+    P->d_jtab = nullptr;
+  }
   P->ndesc = static_cast<int>(P->host_desc.size());
   if (P->ndesc > 0) {
     hipError_t e = hipMalloc(&P->d_desc, P->host_desc.size() * sizeof(bjob));
@@ -677,6 +706,19 @@ upload_descriptors(frsz2_batch_plan* P)
     e = hipMemcpy(P->d_desc, P->host_desc.data(), P->host_desc.size() * sizeof(bjob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       return hip_status(e, "hipMemcpy");
+    }
+    if (P->cfg.tile_shape == 3 && P->total_tiles > 0) {
+      // one workgroup per tile: the tile's job by table (4 bytes per tile of
+      // at least 1 KiB of data) instead of a search over thousands of jobs
+      const std::vector<std::uint32_t> tab = tile_table(P->host_desc, P->total_tiles);
+      e = hipMalloc(&P->d_jtab, tab.size() * sizeof(std::uint32_t));
+      if (e != hipSuccess) {
+        return hip_status(e, "hipMalloc");
+      }
+      e = hipMemcpy(P->d_jtab, tab.data(), tab.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        return hip_status(e, "hipMemcpy");
+      }
     }
   }
   return FRSZ2_OK;
@@ -697,7 +739,7 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
   }
   const batch_shape shape{ P->cfg.tile_shape, P->cfg.workgroups, P->cfg.tiles_per_claim, P->ops };
   const hipError_t e =
-    launch_persistent(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, P->d_counter, shape, s);
+    launch_persistent(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, P->d_counter, shape, s, P->d_jtab);
   if (e != hipSuccess) {
     return hip_status(e, "batch launch");
   }
@@ -788,6 +830,9 @@ destroy(frsz2_batch_plan* P)
   drop_graph(P);
   if (P->d_desc) {
     (void)hipFree(P->d_desc);
+  }
+  if (P->d_jtab) {
+    (void)hipFree(P->d_jtab);
   }
   if (P->d_counter) {
     (void)hipFree(P->d_counter);
