@@ -285,7 +285,7 @@ def bench_host(x_host, bits, block, want_stream, want_y, reps=3):
             "bit_exact_vs_device_path": same}
 
 
-def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499):
+def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_options=None):
     """Poisson arrivals at `rate` requests/s for `seconds`: 50% compress, 30%
     full decompress, 20% random-access reads of 1-4096 values from resident
     streams; sizes log-uniform 4 KiB-256 MiB, float32 / float64 (half each),
@@ -361,7 +361,9 @@ def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499):
     trace["arrival"] = arrival
     torch.cuda.synchronize()
     print(f"bench: serving {nreq} requests over {seconds:.0f} s", file=sys.stderr, flush=True)
-    srv = serve.Server(hbm_budget=budget_gib << 30, slots=4, tile_shape=0, tiles_per_claim=4)
+    opts = dict(slots=4, tile_shape=3)
+    opts.update(server_options or {})
+    srv = serve.Server(hbm_budget=budget_gib << 30, **opts)
     res, tickets = srv.replay(trace)
     stats = srv.stats()
     lat = (res["t_done"] - res["t_arrival"]) * 1e3

@@ -54,8 +54,9 @@ std::uint64_t build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_s
 constexpr int INLINE_JOBS = 8;
 
 // True when launch_persistent passes the descriptors in the kernel arguments
-// (a grid of one workgroup per tile, at most INLINE_JOBS descriptors, not
-// tile shape 3): d_desc is then not read and need not hold them.
+// (at most INLINE_JOBS descriptors, and a grid of one workgroup per tile:
+// tile shape 3, or a persistent grid no larger than the tile count): d_desc
+// is then not read and need not hold them.
 bool descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tiles);
 
 // Counter reset (when claiming) + persistent kernel over ndesc descriptors in

@@ -355,10 +355,8 @@ k_reset_counter(unsigned long long* counter)
 // Grouped mode: one workgroup per tile of the jobs of one class (jobs[0..njobs)
 // are that class's descriptors, tiles tile_base + blockIdx.x).
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
-__global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs,
-                                                 int njobs,
-                                                 std::uint64_t tile_base,
-                                                 const std::uint32_t* __restrict__ jtab)
+__device__ __forceinline__ void
+grouped_tile(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base, const std::uint32_t* __restrict__ jtab)
 {
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
   __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
@@ -388,9 +386,28 @@ __global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs,
   }
 }
 
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
+__global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs,
+                                                 int njobs,
+                                                 std::uint64_t tile_base,
+                                                 const std::uint32_t* __restrict__ jtab)
+{
+  grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(jobs, njobs, tile_base, jtab);
+}
+
+// the class's descriptors (at most INLINE_JOBS) in the kernel arguments
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
+__global__ __launch_bounds__(WGS) void k_grouped_inline(job_pack pack, int njobs, std::uint64_t tile_base)
+{
+  grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(pack.j, njobs, tile_base, nullptr);
+}
+
+// h: host copy of the class's descriptors when they go in the kernel
+// arguments (nd <= INLINE_JOBS), else null
 template<int CLS>
 void
-launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, const std::uint32_t* jtab, hipStream_t s)
+launch_class(const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64_t nt, const std::uint32_t* jtab,
+             hipStream_t s)
 {
   using T = std::conditional_t<cls_dtype(CLS) == 0, float, double>;
   constexpr int U = 8 << cls_ui(CLS);
@@ -401,8 +418,15 @@ launch_class(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, const st
   constexpr bool NT = venc_nt(V) != 0;
   constexpr bool STAGE = OP == 1 && venc_stage(V) != 0;
   constexpr int WGS = class_wgs(CLS, 3);
-  hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)), dim3(WGS),
-                     0, s, d, nd, t0, jtab);
+  if (h != nullptr) {
+    job_pack pack{};
+    std::copy(h, h + nd, pack.j);
+    hipLaunchKernelGGL((k_grouped_inline<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)),
+                       dim3(WGS), 0, s, pack, nd, t0);
+  } else {
+    hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)),
+                       dim3(WGS), 0, s, d, nd, t0, jtab);
+  }
 }
 
 int
@@ -428,7 +452,7 @@ aligned16(const void* p)
 // (descriptors are built in class order)
 hipError_t
 launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, const std::uint32_t* jtab,
-               hipStream_t s)
+               bool inl, hipStream_t s)
 {
   int d0 = 0;
   while (d0 < nd) {
@@ -442,7 +466,7 @@ launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t tot
     if (t1 > t0) {
 #define FRSZ2_G(K)                                                                                           \
   case K:                                                                                                    \
-    launch_class<K>(d_desc + d0, d1 - d0, t0, t1 - t0, jtab, s);                                             \
+    launch_class<K>(d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s);                \
     break
       switch (cls) {
         FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
@@ -586,8 +610,10 @@ persistent_grid(const batch_shape& shape, std::uint64_t total_tiles)
 bool
 descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tiles)
 {
-  return shape.tile_shape != 3 && ndesc > 0 && ndesc <= INLINE_JOBS && total_tiles > 0 &&
-         persistent_grid(shape, total_tiles) == total_tiles;
+  if (ndesc <= 0 || ndesc > INLINE_JOBS || total_tiles == 0) {
+    return false;
+  }
+  return shape.tile_shape == 3 || persistent_grid(shape, total_tiles) == total_tiles;
 }
 
 hipError_t
@@ -598,7 +624,8 @@ launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64
     return hipSuccess;
   }
   if (shape.tile_shape == 3) {
-    return launch_grouped(d_desc, h_desc, ndesc, total_tiles, d_jtab, s);
+    return launch_grouped(d_desc, h_desc, ndesc, total_tiles, d_jtab,
+                          h_desc != nullptr && descriptors_inline(shape, ndesc, total_tiles), s);
   }
   const std::uint64_t grid = persistent_grid(shape, total_tiles);
   // a grid that covers every tile takes one tile per workgroup: no claims,

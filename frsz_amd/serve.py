@@ -89,7 +89,7 @@ def result_dict(r: Result) -> dict:
 
 
 class Server:
-    def __init__(self, hbm_budget: int = 64 << 30, slots: int = 4, max_batch: int = 1024, tile_shape: int = 0,
+    def __init__(self, hbm_budget: int = 64 << 30, slots: int = 4, max_batch: int = 1024, tile_shape: int = 3,
                  workgroups: int = 0, tiles_per_claim: int = 4):
         cfg = ServerConfig(hbm_budget, slots, max_batch, tile_shape, workgroups, tiles_per_claim, 0)
         h = ctypes.c_void_p()

@@ -81,11 +81,13 @@ def test_batch_matches_plain(fz, cfg):
 
 @pytest.mark.parametrize("count,cfg", [(1, None), (3, {"use_graph": 0}), (8, {"use_graph": 1}),
                                        (9, {"use_graph": 0}), (5, {"workgroups": 16, "tiles_per_claim": 2}),
-                                       (4, {"workgroups": 16, "tiles_per_claim": 0, "use_graph": 1})])
+                                       (4, {"workgroups": 16, "tiles_per_claim": 0, "use_graph": 1}),
+                                       (3, {"tile_shape": 3, "use_graph": 0}), (8, {"tile_shape": 3, "use_graph": 1}),
+                                       (9, {"tile_shape": 3, "use_graph": 1})])
 def test_few_jobs(fz, count, cfg):
-    """Up to eight descriptors over a grid of one workgroup per tile travel in
-    the kernel arguments (no descriptor copy); nine, or a grid smaller than
-    the tile count, read them from device memory."""
+    """Up to eight descriptors over a grid of one workgroup per tile (or the
+    grouped shape) travel in the kernel arguments (no descriptor copy); nine,
+    or a grid smaller than the tile count, read them from device memory."""
     fz_mod, batch = fz
     specs = make_arrays(30 + count, count=count)
     ref_s, ref_v = plain_outputs(fz_mod, specs)

@@ -35,11 +35,12 @@ def arrays(seed, count):
     return out
 
 
-def test_mix_equals_alone(mods):
+@pytest.mark.parametrize("tile_shape", [3, 0, 2])
+def test_mix_equals_alone(mods, tile_shape):
     fz, serve = mods
     specs = arrays(1, 40)
     rng = np.random.default_rng(2)
-    with serve.Server(hbm_budget=1 << 30, slots=3) as srv:
+    with serve.Server(hbm_budget=1 << 30, slots=3, tile_shape=tile_shape) as srv:
         comp, dec, gat = [], [], []
         for x, bits, B in specs:
             t = srv.submit_compress(x, bits, B)
