@@ -402,7 +402,9 @@ template<typename T, int U>
 __device__ __forceinline__ T
 decode_element(const std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t idx)
 {
-  const std::uint64_t k = idx / L.B;
+  // a shift for power-of-two B (the branch is uniform when L is), else a
+  // 64-bit division
+  const std::uint64_t k = (L.B & (L.B - 1)) == 0 ? idx >> __builtin_ctzll(L.B) : idx / L.B;
   const std::uint64_t j = idx - k * L.B;
   const int E = load_block_exponent<U>(stream, k);
   const std::uint64_t code = load_code<U>(stream + L.R + k * L.CB, j, L.bits);
