@@ -66,3 +66,47 @@ def test_field_1gib_full_size(port_mt):
     err = (x - y).abs().view(-1, 32).amax(dim=1)
     E = torch.from_numpy(want[: x.numel() // 32 * 4].view(np.int32).astype(np.int64)).cuda()
     assert bool((err < torch.exp2((E - 30).double())).all())
+
+
+@pytest.mark.parametrize("bits,B", [(12, 32), (32, 64)])
+def test_more_than_2p32_values(port, bits, B):
+    """Maximum sizes: n = 2^32 + 4133 float32 values (17 GB in, value and byte
+    offsets past 32 bits).  Blocks are independent, so block k of the stream
+    equals the one-block stream of its values: checked by the oracle for the
+    first blocks, those around value 2^32 and the partial last block, with the
+    exponent pad, the decoded values there and gathers past 2^32."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frsz_amd
+    from oracle.oracle import compressed_bytes, storage_bits
+
+    n = (1 << 32) + 4133
+    print(f"2^32+ values bits={bits} B={B}", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(bits)
+    x = torch.randn(n, device="cuda", generator=g) * 1e3
+    s = frsz_amd.compress(x, bits, B)
+    assert s.numel() == compressed_bytes(bits, B, n)
+    y = frsz_amd.decompress(s, n, torch.float32, bits, B)
+    U8 = storage_bits(bits) // 8
+    nb = -(-n // B)
+    cb = -(-(B * bits) // (U8 * 8)) * U8
+    R = s.numel() - nb * cb
+    assert not torch.any(s[nb * U8:R]), "exponent pad"
+    mid = (1 << 32) // B
+    for k in [0, 1, mid - 2, mid - 1, mid, mid + 1, nb - 2, nb - 1]:
+        v0, v1 = k * B, min(n, (k + 1) * B)
+        vals = x[v0:v1].cpu().numpy()
+        one = port.compress(vals, bits, B)  # the one-block stream of these values
+        got_e = s[k * U8:(k + 1) * U8].cpu().numpy()
+        got_p = s[R + k * cb:R + (k + 1) * cb].cpu().numpy()
+        assert np.array_equal(got_e, one[:U8]), ("exponent", k)
+        assert np.array_equal(got_p, one[256:256 + cb]), ("payload", k)
+        want_y = port.decompress(one, v1 - v0, np.float32, bits, B)
+        assert np.array_equal(y[v0:v1].cpu().numpy().view(np.uint32), want_y.view(np.uint32)), ("values", k)
+    idx = torch.tensor([0, (1 << 32) - 1, 1 << 32, (1 << 32) + 1, n - 1, n, (1 << 33)], dtype=torch.int64,
+                       device="cuda")
+    gv = frsz_amd.gather(s, n, torch.float32, bits, B, idx).cpu().numpy().view(np.uint32)
+    yv = y[idx[:5]].cpu().numpy().view(np.uint32)
+    assert np.array_equal(gv[:5], yv)
+    assert np.array_equal(gv[5:], np.zeros(2, np.uint32)), "indices >= n gather 0"
