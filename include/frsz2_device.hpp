@@ -63,8 +63,15 @@ __host__ __device__ __forceinline__ std::uint32_t
 max_key(typename fmt<T>::uw u)
 {
   using F = fmt<T>;
-  const typename F::uw a = u & F::ABS;
-  return a > F::INF ? 0u : static_cast<std::uint32_t>(a >> F::SB);
+  if constexpr (sizeof(T) == 4) {
+    // NaN as an unordered float compare (one v_cmp_u_f32), the exponent
+    // field with one v_bfe
+    const float f = __builtin_bit_cast(float, u);
+    return f != f ? 0u : static_cast<std::uint32_t>((u >> F::SB) & 0xffu);
+  } else {
+    const typename F::uw a = u & F::ABS;
+    return a > F::INF ? 0u : static_cast<std::uint32_t>(a >> F::SB);
+  }
 }
 
 // Block exponent as stored, from the block's max key: exponent() returns
