@@ -139,8 +139,20 @@ encode_value_short(typename fmt<T>::uw u, int E, int bits)
   if (sizeof(T) == 4) {
     e = static_cast<std::int8_t>(static_cast<std::uint8_t>(e));
   }
-  const uw m = (u & ((uw{ 1 } << F::SB) - 1)) | (biased != 0 ? (uw{ 1 } << F::SB) : uw{ 0 });
-  const int tot = (F::W - bits) - (F::EB - 1) + (E - e);
+  uw m;
+  if constexpr (sizeof(T) == 4) {
+    // the implicit bit unless |x| is zero or subnormal, as one float compare
+    // (NaN compares false and keeps the bit; a compare that flushes
+    // subnormals to zero gives the same answer)
+    const bool low = __builtin_fabsf(__builtin_bit_cast(float, u)) < 0x1p-126f;
+    m = (u & ((uw{ 1 } << F::SB) - 1)) | (low ? uw{ 0 } : (uw{ 1 } << F::SB));
+  } else {
+    m = (u & ((uw{ 1 } << F::SB) - 1)) | (biased != 0 ? (uw{ 1 } << F::SB) : uw{ 0 });
+  }
+  // the per-block part of the folded shift first, so that one subtraction
+  // per value remains
+  const int k = (F::W - bits) - (F::EB - 1) + E;
+  const int tot = k - e;
   const uw mag = m >> (tot < F::W - 1 ? tot : F::W - 1);
   const uw s = u >> (F::W - 1);
   return mag | (s << (bits - 1));
