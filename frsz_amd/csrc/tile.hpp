@@ -740,8 +740,10 @@ stage_dwords()
   return WGS * VPT * static_cast<int>(sizeof(T)) / 4;
 }
 
-// Decompress one tile of packed codes (bits < U), strided: thread t decodes
-// tile values k*WGS + t (k < VPT).  The bit offset of tile value i in the
+// Decompress one tile, strided: thread t decodes tile values k*WGS + t
+// (k < VPT).  Packed codes (bits < U) are read from an LDS image of the
+// tile's payload; codes of bits == U are loaded from HBM directly (word i of
+// the payload is value i).  The bit offset of tile value i in the
 // tile's payload is f(i) = (i >> lb)*CB*8 + (i & (B-1))*bits, and for
 // power-of-two B, f(k*WGS + t) = f(k*WGS) + f(t) (B >= WGS: k*WGS mod B + t
 // < B; B < WGS: k*WGS is a multiple of B), so a value's offset is one add of
@@ -749,7 +751,7 @@ stage_dwords()
 // of the LDS image (no bank conflicts, whatever bits is) and stores 64
 // consecutive values, so no output staging is needed.  A value's exponent
 // is loaded for it (lanes of one block read one address).
-template<typename T, int U, int VPT, bool NT, int WGS>
+template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS>
 __device__ __forceinline__ void
 decompress_tile_strided(const std::uint8_t* __restrict__ in,
                         typename fmt<T>::uw* __restrict__ out,
@@ -789,31 +791,43 @@ decompress_tile_strided(const std::uint8_t* __restrict__ in,
     }
   }
 
-  const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
-  const std::uint64_t left = p.nb - k0;
-  const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
-  const std::uint32_t nbytes = blocks * p.CB;
-  const std::uint8_t* src = in + p.R + k0 * p.CB;
-  const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
-  u32x4* l4 = reinterpret_cast<u32x4*>(lds);
-  for (std::uint32_t c = t; c < nbytes / 16; c += WGS) {
-    l4[c] = ld<NT>(s4 + c);
-  }
-  const std::uint32_t tail = nbytes & 15u;
-  if (t < tail) {
-    const std::uint32_t o = nbytes - tail + t;
-    reinterpret_cast<std::uint8_t*>(lds)[o] = src[o];
-  }
-  __syncthreads();
-
-  const std::uint32_t P = (t >> p.lb) * cb8 + (t & bm) * bits;
   uw code[VPT];
+  if constexpr (PACKED) {
+    const std::uint32_t TB = static_cast<std::uint32_t>(TILE >> p.lb);
+    const std::uint64_t left = p.nb - k0;
+    const std::uint32_t blocks = left < TB ? static_cast<std::uint32_t>(left) : TB;
+    const std::uint32_t nbytes = blocks * p.CB;
+    const std::uint8_t* src = in + p.R + k0 * p.CB;
+    const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
+    u32x4* l4 = reinterpret_cast<u32x4*>(lds);
+    for (std::uint32_t c = t; c < nbytes / 16; c += WGS) {
+      l4[c] = ld<NT>(s4 + c);
+    }
+    const std::uint32_t tail = nbytes & 15u;
+    if (t < tail) {
+      const std::uint32_t o = nbytes - tail + t;
+      reinterpret_cast<std::uint8_t*>(lds)[o] = src[o];
+    }
+    __syncthreads();
+
+    const std::uint32_t P = (t >> p.lb) * cb8 + (t & bm) * bits;
 #pragma unroll
-  for (int k = 0; k < VPT; ++k) {
-    const std::uint32_t kw = static_cast<std::uint32_t>(k * WGS);
-    // uniform; kept whole so the per-value work is the one add
-    const std::uint32_t sk = __builtin_amdgcn_readfirstlane((kw >> p.lb) * cb8 + (kw & bm) * bits);
-    code[k] = static_cast<uw>(extract_code<U>(lds, P + sk));
+    for (int k = 0; k < VPT; ++k) {
+      const std::uint32_t kw = static_cast<std::uint32_t>(k * WGS);
+      // uniform; kept whole so the per-value work is the one add
+      const std::uint32_t sk = __builtin_amdgcn_readfirstlane((kw >> p.lb) * cb8 + (kw & bm) * bits);
+      code[k] = static_cast<uw>(extract_code<U>(lds, P + sk));
+    }
+  } else {
+    // bits == U: payload word i holds value i (words up to nb*B exist)
+    using wt = typename word<U>::type;
+    const wt* w = reinterpret_cast<const wt*>(in + p.R) + base;
+    const std::uint64_t limit = p.nb << p.lb;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const std::uint32_t kw = static_cast<std::uint32_t>(k * WGS);
+      code[k] = full || base + kw + t < limit ? static_cast<uw>(ld<NT>(w + kw + t)) : uw{ 0 };
+    }
   }
   // decode_value_short needs bits <= SB + 1 and E + bias >= 0 (every block a
   // compress writes); the check is per thread, over all of its values
@@ -849,30 +863,25 @@ decompress_tile_strided(const std::uint8_t* __restrict__ in,
       }
     }
   }
-  __syncthreads();
+  if constexpr (PACKED) {
+    __syncthreads(); // the LDS image is reused by the next tile of a persistent workgroup
+  }
 }
 
-template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int WGS = WG>
+template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS>
 __device__ __forceinline__ void
-decompress_tile(const std::uint8_t* __restrict__ in,
-                typename fmt<T>::uw* __restrict__ out,
-                const tplan& p,
-                std::uint64_t tile,
-                std::uint32_t* lds,
-                std::uint32_t* stage)
+decompress_tile_chunked(const std::uint8_t* __restrict__ in,
+                        typename fmt<T>::uw* __restrict__ out,
+                        const tplan& p,
+                        std::uint64_t tile,
+                        std::uint32_t* lds,
+                        std::uint32_t* stage)
 {
-  if constexpr (PACKED && !STAGE) {
-    decompress_tile_strided<T, U, VPT, NT, WGS>(in, out, p, tile, lds);
-    return;
-  }
   using uw = typename fmt<T>::uw;
   constexpr int TILE = WGS * VPT;
-  // values per chunk.  Staged: 16 bytes of codes (or all of a thread's
-  // values when the codes come from the LDS image), restaged through LDS for
-  // whole-line stores.  Direct: one 16-byte output store per chunk, so each
-  // store instruction of a wave covers 1 KiB of consecutive output.
-  constexpr int OUTV = 16 / static_cast<int>(sizeof(T));
-  constexpr int VC = STAGE ? (PACKED ? VPT : (VPT < 128 / U ? VPT : 128 / U)) : (VPT < OUTV ? VPT : OUTV);
+  // values per chunk: 16 bytes of codes (or all of a thread's values when the
+  // codes come from the LDS image), restaged through LDS for whole-line stores
+  constexpr int VC = PACKED ? VPT : (VPT < 128 / U ? VPT : 128 / U);
   constexpr int NCH = VPT / VC;
   constexpr int NB = VC * U / 8; // code bytes per chunk
   static_assert(VC * static_cast<int>(sizeof(T)) % 16 == 0, "a chunk is whole 16-byte stores");
@@ -969,54 +978,50 @@ decompress_tile(const std::uint8_t* __restrict__ in,
       }
     }
   }
-  if constexpr (STAGE) {
-    uw* sv = reinterpret_cast<uw*>(stage);
+  uw* sv = reinterpret_cast<uw*>(stage);
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      const std::uint32_t local0 = static_cast<std::uint32_t>((c * WGS + t) * VC);
-      uw y[VC];
-      decode_chunk<T, VC>(code[c], E[c], uniform, narrow, p.bits, y);
-      store_values<T, VC, false>(sv + local0, y);
-    }
-    __syncthreads();
-    constexpr int CPV = 16 / static_cast<int>(sizeof(T));
-    constexpr int NQ = TILE / CPV; // 16-byte chunks of the tile
-    const std::uint64_t left = p.n - base;
-    const std::uint32_t nvals = left < static_cast<std::uint64_t>(TILE) ? static_cast<std::uint32_t>(left) : TILE;
-    const u32x4* s4 = reinterpret_cast<const u32x4*>(stage);
-    u32x4* o4 = reinterpret_cast<u32x4*>(out + base);
+  for (int c = 0; c < NCH; ++c) {
+    const std::uint32_t local0 = static_cast<std::uint32_t>((c * WGS + t) * VC);
+    uw y[VC];
+    decode_chunk<T, VC>(code[c], E[c], uniform, narrow, p.bits, y);
+    store_values<T, VC, false>(sv + local0, y);
+  }
+  __syncthreads();
+  constexpr int CPV = 16 / static_cast<int>(sizeof(T));
+  constexpr int NQ = TILE / CPV; // 16-byte chunks of the tile
+  const std::uint64_t left = p.n - base;
+  const std::uint32_t nvals = left < static_cast<std::uint64_t>(TILE) ? static_cast<std::uint32_t>(left) : TILE;
+  const u32x4* s4 = reinterpret_cast<const u32x4*>(stage);
+  u32x4* o4 = reinterpret_cast<u32x4*>(out + base);
 #pragma unroll
-    for (int q0 = 0; q0 < NQ; q0 += WGS) {
-      const std::uint32_t q = static_cast<std::uint32_t>(q0 + t);
-      if ((q + 1) * CPV <= nvals) {
-        st<NT>(s4[q], o4 + q);
-      } else if (q * CPV < nvals) {
-        for (std::uint32_t v = q * CPV; v < nvals; ++v) {
-          out[base + v] = sv[v];
-        }
-      }
-    }
-    __syncthreads();
-  } else {
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      const std::uint64_t i0 = base + static_cast<std::uint32_t>((c * WGS + t) * VC);
-      uw y[VC];
-      decode_chunk<T, VC>(code[c], E[c], uniform, narrow, p.bits, y);
-      if (i0 + VC <= p.n) {
-        store_values<T, VC, NT>(out + i0, y);
-      } else {
-#pragma unroll
-        for (int v = 0; v < VC; ++v) {
-          if (i0 + v < p.n) {
-            out[i0 + v] = y[v];
-          }
-        }
+  for (int q0 = 0; q0 < NQ; q0 += WGS) {
+    const std::uint32_t q = static_cast<std::uint32_t>(q0 + t);
+    if ((q + 1) * CPV <= nvals) {
+      st<NT>(s4[q], o4 + q);
+    } else if (q * CPV < nvals) {
+      for (std::uint32_t v = q * CPV; v < nvals; ++v) {
+        out[base + v] = sv[v];
       }
     }
   }
-  if (PACKED) {
-    __syncthreads();
+  __syncthreads(); // the stage and the payload image are reused by the next tile of a persistent workgroup
+}
+
+// Decompress one tile: staged, the chunked decode with LDS-staged stores;
+// otherwise the strided decode.
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int WGS = WG>
+__device__ __forceinline__ void
+decompress_tile(const std::uint8_t* __restrict__ in,
+                typename fmt<T>::uw* __restrict__ out,
+                const tplan& p,
+                std::uint64_t tile,
+                std::uint32_t* lds,
+                std::uint32_t* stage)
+{
+  if constexpr (STAGE) {
+    decompress_tile_chunked<T, U, VPT, PACKED, NT, WGS>(in, out, p, tile, lds, stage);
+  } else {
+    decompress_tile_strided<T, U, VPT, PACKED, NT, WGS>(in, out, p, tile, lds);
   }
 }
 

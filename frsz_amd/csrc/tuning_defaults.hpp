@@ -6,9 +6,9 @@
 
 namespace frsz2 {
 
-// encoded variant: values per thread, staged decompress (packed codes: the
-// chunked decode with LDS-staged stores; off: the strided decode), the
-// non-temporal hint, linear (workgroup-order) tiles, 64-thread workgroups
+// encoded variant: values per thread, staged decompress (the chunked decode
+// with LDS-staged stores; off: the strided decode), the non-temporal hint,
+// linear (workgroup-order) tiles, 64-thread workgroups
 constexpr int
 venc(int vpt, int stage, int nt, int linear = 0, int small = 0)
 {

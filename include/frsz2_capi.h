@@ -161,10 +161,10 @@ int frsz2_decompress_blocks(int dtype,
  * only; otherwise taken as 4), 4, 8 or 16: a tile is 256 x that many values
  * and is raised to the next size that holds a whole exponent block;
  * flags: FRSZ2_TUNE_NONTEMPORAL puts the non-temporal hint on the streaming
- * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) writes each tile's output
- * through LDS with whole-cache-line stores (for bits < U it selects the
- * decode of consecutive values per thread over the strided decode, which
- * stores straight from registers), FRSZ2_TUNE_LINEAR_ORDER takes
+ * accesses, FRSZ2_TUNE_STAGED_STORES (decompress) selects the decode of
+ * consecutive values per thread whose tile output goes through LDS with
+ * whole-cache-line stores (without it: the strided decode, one value per
+ * lane per step, stored straight from registers), FRSZ2_TUNE_LINEAR_ORDER takes
  * tiles in workgroup order (default: one contiguous range of tiles per
  * XCD), FRSZ2_TUNE_SMALL_WORKGROUPS launches 64-thread workgroups (one wave,
  * tiles of 64 x values_per_thread; implies the non-temporal hint) instead of
