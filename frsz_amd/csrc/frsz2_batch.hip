@@ -154,9 +154,8 @@ class_stage(int cls, int shape)
   if (cls / 14 != 1 || cls >= 28 || packed) {
     return false;
   }
-  if (shape <= 1) {
-    return !f32;
-  }
+  // bits == U: the strided decode loads one code per lane per step, which
+  // pays off for 4-byte codes (float32, U 32) and not for 1-2-byte ones
   return !(f32 && slot / 2 == 2);
 }
 
