@@ -11,7 +11,7 @@ import pytest
 
 from conftest import GOLDEN, GOLDEN_CASES
 from frsz_amd.workloads import smooth_signal
-from oracle.oracle import compressed_bytes
+from oracle.oracle import compressed_bytes, storage_bits
 
 
 def f32(bits):
@@ -180,6 +180,31 @@ def test_oracle_matches_compiled_reference(port, reference, dtype):
             assert np.array_equal(a, port.compress(x, bits, B)), (bits, B, n)
             ya = reference.decompress(a, n, dtype, bits, B)
             assert np.array_equal(ya.view(np.uint8), port.decompress(a, n, dtype, bits, B).view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_oracle_decodes_arbitrary_streams_like_compiled_reference(port, reference, dtype):
+    """Streams no compress writes (random payload bytes; block exponents over
+    the range where the reference's decode is defined: its shift counts stay
+    below the width, E + bias >= -24 for float32 and >= -53 for float64, up to
+    40000), which the GPU parity tests decode against the oracle."""
+    rng = np.random.default_rng(12)
+    W = np.dtype(dtype).itemsize * 8
+    for bits in (4, 7, 8, 9, 12, 16, 17, 21, 24, 31, 32) + ((33, 40, 52, 53, 63, 64) if W == 64 else ()):
+        for B in (1, 5, 16, 32, 64):
+            n = int(rng.integers(1, 3000))
+            U = storage_bits(bits)
+            nb = -(-n // B)
+            s = rng.integers(0, 256, compressed_bytes(bits, B, n), dtype=np.uint8)
+            et = {8: np.int8, 16: np.int16, 32: np.int32, 64: np.int64}[U]
+            lo = max(int(np.iinfo(et).min), -151 if W == 32 else -1076)
+            hi = min(int(np.iinfo(et).max), 40_000)
+            E = rng.integers(lo, hi, nb, endpoint=True).astype(et)
+            E[: min(nb, 3)] = np.array([lo, hi, -1], dtype=et)[: min(nb, 3)]
+            s[: nb * U // 8] = E.view(np.uint8)
+            a = reference.decompress(s, n, dtype, bits, B)
+            b = port.decompress(s, n, dtype, bits, B)
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (bits, B, n)
 
 
 def test_value_helpers_match_compiled_reference(port, reference):
