@@ -95,7 +95,7 @@ class BatchPlan:
         handle = ctypes.c_void_p()
         cfg = None
         if config is not None:
-            cfg = BatchConfig(config.get("workgroups", 0), config.get("tile_shape", 1),
+            cfg = BatchConfig(config.get("workgroups", 0), config.get("tile_shape", 3),
                               config.get("tiles_per_claim", 4), config.get("use_graph", 1),
                               config.get("plain_launches", 0))
         check(_L.frsz2_batch_create(arr, len(jobs), ctypes.byref(cfg) if cfg is not None else None,

@@ -831,7 +831,7 @@ apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
 {
   frsz2_batch_config n = c;
   if (n.tile_shape < 0 || n.tile_shape > 3) {
-    n.tile_shape = 1;
+    n.tile_shape = 3;
   }
   if (n.workgroups <= 0) {
     n.workgroups = 4 * P->cus;
@@ -902,7 +902,7 @@ frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config
     c = *cfg;
   } else {
     c.workgroups = 0;
-    c.tile_shape = 1;
+    c.tile_shape = 3;
     c.tiles_per_claim = 4;
     c.use_graph = 1;
     c.plain_launches = 0;

@@ -603,7 +603,7 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
     S->cfg.max_batch = 1024;
   }
   if (S->cfg.tile_shape < 0 || S->cfg.tile_shape > 3) {
-    S->cfg.tile_shape = 0;
+    S->cfg.tile_shape = 3;
   }
   if (S->cfg.workgroups <= 0) {
     S->cfg.workgroups = 4 * device_cus();

@@ -242,7 +242,12 @@ typedef struct frsz2_batch_config
                             16 values per thread, staged decompress stores);
                             3 grouped: instead of one persistent kernel, one
                             launch per kernel class present (same graph), each
-                            with that class's plain-kernel default variant */
+                            with that class's plain-kernel default variant
+                            and one workgroup per tile (the default: the
+                            persistent shapes keep one tile in flight per
+                            workgroup and stream at 2-4 TB/s, the grouped
+                            launches at the plain kernels' rate); out of
+                            range = 3 */
   int tiles_per_claim;   /* tiles per atomic claim; 0 = static round-robin */
   int use_graph;         /* 1 = launch through a captured hipGraph */
   int plain_launches;    /* 1 = no persistent kernel: one plain frsz2_compress /
@@ -253,8 +258,8 @@ typedef struct frsz2_batch_config
 typedef struct frsz2_batch_plan frsz2_batch_plan;
 
 /* Validate the jobs, upload their descriptors and capture the launch.
- * cfg NULL = defaults (4 workgroups per CU, balanced tiles, 4 tiles per
- * claim, graph on).  Jobs must not overlap in memory. */
+ * cfg NULL = defaults (grouped launches, graph on; for the persistent
+ * shapes 4 workgroups per CU and 4 tiles per claim).  Jobs must not overlap in memory. */
 int frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config* cfg,
                        frsz2_batch_plan** plan);
 /* Run every job of the plan, stream-ordered.  Launches of one plan must not
@@ -273,8 +278,8 @@ int frsz2_batch_destroy(frsz2_batch_plan* plan);
  * array).  A server owns a device pool of hbm_budget bytes from which every
  * request's output is allocated; a scheduler thread admits requests in
  * arrival order while their outputs fit, and runs each group of admitted
- * requests as one launch of the persistent batch kernel on one of `slots`
- * streams, so requests arriving while others run are admitted into the next
+ * requests as one batch launch (the kernels of frsz2_batch_launch) on one of
+ * `slots` streams, so requests arriving while others run are admitted into the next
  * launch.  A request computes exactly what frsz2_compress / frsz2_decompress
  * / frsz2_gather compute. */
 
@@ -288,7 +293,7 @@ typedef struct frsz2_server_config
   int tile_shape;      /* batch kernel tile shape (see frsz2_batch_config); 3
                           (one grouped launch per class, one workgroup per
                           tile) gave a 2.2x lower p99 than 0 on a mixed
-                          4 KiB-256 MiB request stream */
+                          4 KiB-256 MiB request stream; out of range = 3 */
   int workgroups;      /* persistent grid (0 = 4 per CU) */
   int tiles_per_claim; /* 0 = static round-robin */
   int stall_ms;        /* when nothing runs and the oldest queued request has
