@@ -62,16 +62,17 @@ bool descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total
 // Counter reset (when claiming) + persistent kernel over ndesc descriptors in
 // device memory (d_desc; h_desc is the host copy, which is what the kernel
 // reads when descriptors_inline holds), on stream s.  Tile shape 3 launches
-// one grouped kernel per class instead, finding each tile's job in d_jtab
-// (tile_table of the descriptors, in device memory) or, when that is null,
-// by a binary search.
+// one grouped kernel per class instead.  Each tile's job is read from d_jtab
+// (tile_table of the descriptors for this tile shape, in device memory) or,
+// when that is null, found by a binary search.
 hipError_t launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
                              unsigned long long* d_counter, const batch_shape& shape, hipStream_t s,
                              const std::uint32_t* d_jtab = nullptr);
 
-// For tile shape 3: entry t is the index, within its class's run of
-// descriptors, of the descriptor that owns global tile t.
-std::vector<std::uint32_t> tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles);
+// Entry t is the index of the descriptor that owns global tile t: within its
+// class's run of descriptors (tile shape 3, whose kernels see one class's
+// descriptors) or among all of them (the persistent shapes).
+std::vector<std::uint32_t> tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles, bool within_class);
 
 // One job as a plain single-array launch on s.
 int run_plain(const frsz2_job& j, hipStream_t s);

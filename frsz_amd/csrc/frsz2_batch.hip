@@ -281,11 +281,12 @@ find_job(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, int hint)
 }
 
 // The persistent loop: tiles claimed `chunk` at a time from *counter, or
-// (chunk == 0) taken round-robin by workgroup index.
+// (chunk == 0) taken round-robin by workgroup index.  A tile's job comes from
+// jtab (descriptor index per tile) when given, else from a search.
 template<int SHAPE, int OPS>
 __device__ __forceinline__ void
 batch_loop(const bjob* __restrict__ jobs, int njobs, std::uint64_t total_tiles, unsigned long long* counter,
-           int chunk)
+           int chunk, const std::uint32_t* __restrict__ jtab)
 {
   __shared__ std::uint32_t lds[batch_lds_dwords<SHAPE>()];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
@@ -315,7 +316,7 @@ batch_loop(const bjob* __restrict__ jobs, int njobs, std::uint64_t total_tiles, 
       g1 = total_tiles;
     }
     for (std::uint64_t g = g0; g < g1; ++g) {
-      j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, j));
+      j = __builtin_amdgcn_readfirstlane(jtab ? static_cast<int>(jtab[g]) : find_job(jobs, njobs, g, j));
       const bjob J = jobs[j];
       run_tile<SHAPE, OPS>(J, g - J.tile0, lds, wred);
     }
@@ -327,9 +328,10 @@ __global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
                                               int njobs,
                                               std::uint64_t total_tiles,
                                               unsigned long long* counter,
-                                              int chunk)
+                                              int chunk,
+                                              const std::uint32_t* __restrict__ jtab)
 {
-  batch_loop<SHAPE, OPS>(jobs, njobs, total_tiles, counter, chunk);
+  batch_loop<SHAPE, OPS>(jobs, njobs, total_tiles, counter, chunk, jtab);
 }
 
 // Up to INLINE_JOBS descriptors passed in the kernel arguments: a launch of
@@ -342,7 +344,7 @@ struct job_pack
 template<int SHAPE, int OPS>
 __global__ __launch_bounds__(WG) void k_batch_inline(job_pack pack, int njobs, std::uint64_t total_tiles)
 {
-  batch_loop<SHAPE, OPS>(pack.j, njobs, total_tiles, nullptr, 0);
+  batch_loop<SHAPE, OPS>(pack.j, njobs, total_tiles, nullptr, 0, nullptr);
 }
 
 __global__ void
@@ -582,13 +584,13 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
 }
 
 std::vector<std::uint32_t>
-tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles)
+tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles, bool within_class)
 {
   std::vector<std::uint32_t> tab(total_tiles, 0u);
   const std::size_t nd = desc.size();
-  std::size_t d0 = 0; // first descriptor of the current class
+  std::size_t d0 = 0; // first descriptor of the current class (0 for global indices)
   for (std::size_t d = 0; d < nd; ++d) {
-    if (desc[d].cls != desc[d0].cls) {
+    if (within_class && desc[d].cls != desc[d0].cls) {
       d0 = d;
     }
     const std::uint64_t t1 = d + 1 < nd ? desc[d + 1].tile0 : total_tiles;
@@ -645,7 +647,8 @@ launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64
   if (inl) {                                                                                                 \
     hipLaunchKernelGGL((k_batch_inline<S, O>), gd, dim3(WG), 0, s, pack, ndesc, total_tiles);                \
   } else {                                                                                                   \
-    hipLaunchKernelGGL((k_batch<S, O>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk);   \
+    hipLaunchKernelGGL((k_batch<S, O>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk,    \
+                       d_jtab);                                                                              \
   }
   switch (shape.tile_shape * 8 + ops) {
     case 0 * 8 + 1: FRSZ2_KB(0, 1); break;
@@ -695,7 +698,7 @@ struct frsz2_batch_plan
   std::vector<char> tiled;            // job index -> 1 if it takes the persistent kernel
   std::vector<frsz2::bjob> host_desc; // descriptors for cfg.tile_shape
   frsz2::bjob* d_desc = nullptr;
-  std::uint32_t* d_jtab = nullptr; // tile shape 3: tile -> descriptor index within its class
+  std::uint32_t* d_jtab = nullptr; // tile -> descriptor index (within its class for tile shape 3)
   unsigned long long* d_counter = nullptr;
   int ndesc = 0;
   std::uint64_t total_tiles = 0;
@@ -733,10 +736,10 @@ upload_descriptors(frsz2_batch_plan* P)
     if (e != hipSuccess) {
       return hip_status(e, "hipMemcpy");
     }
-    if (P->cfg.tile_shape == 3 && P->total_tiles > 0) {
-      // one workgroup per tile: the tile's job by table (4 bytes per tile of
-      // at least 1 KiB of data) instead of a search over thousands of jobs
-      const std::vector<std::uint32_t> tab = tile_table(P->host_desc, P->total_tiles);
+    if (P->total_tiles > 0) {
+      // the tile's job by table (4 bytes per tile of at least 1 KiB of data)
+      // instead of a search over thousands of jobs
+      const std::vector<std::uint32_t> tab = tile_table(P->host_desc, P->total_tiles, P->cfg.tile_shape == 3);
       e = hipMalloc(&P->d_jtab, tab.size() * sizeof(std::uint32_t));
       if (e != hipSuccess) {
         return hip_status(e, "hipMalloc");

@@ -245,7 +245,7 @@ typedef struct frsz2_batch_config
                             with that class's plain-kernel default variant
                             and one workgroup per tile (the default: the
                             persistent shapes keep one tile in flight per
-                            workgroup and stream at 2-4 TB/s, the grouped
+                            workgroup and stream at 3-5 TB/s, the grouped
                             launches at the plain kernels' rate); out of
                             range = 3 */
   int tiles_per_claim;   /* tiles per atomic claim; 0 = static round-robin */
