@@ -10,9 +10,12 @@ compress reads n*8 and writes the stream, decompress the reverse.
                     [--cpu-sample N] [--no-verify]
 
 Prints one JSON line.  ``roofline`` is for the dominant kernel (timed with
-HIP events on the stream the kernels run on); ``cpu_baseline`` times the
-reference CPU codec (oracle/_ref, built from the reference sources) on a
-bounded sample of the same field, single-threaded like the reference.
+HIP events on the stream the kernels run on).  The CPU leg (``cpu_leg``) is
+the only code here that touches the oracle: it checks the GPU outputs of the
+field and of the small case against it (``bit_exact``), and ``cpu_baseline``
+times the reference CPU codec (oracle/_ref, built from the reference sources)
+on a bounded sample of the same field and on the whole small case,
+single-threaded like the reference.
 """
 from __future__ import annotations
 
@@ -48,6 +51,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-edge", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the float32 2^20-value case")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
@@ -56,8 +60,8 @@ def parse():
 
 def bench_batched(count, steps, seed=4, target_input_bytes=None):
     """4096 arrays, lengths log-uniform over [1e3, 1.6e7], 60% float32 /
-    40% float64, bits from {16, 32}, B = 32: one autotuned persistent
-    graph launch vs a C++ loop of plain per-array launches, per direction.
+    40% float64, bits from {16, 32}, B = 32: one autotuned graph launch
+    vs a C++ loop of plain per-array launches, per direction.
     With target_input_bytes every length is divided by one factor so the
     input totals that many bytes (the workload's stated "about 4 GiB")."""
     import numpy as np
@@ -108,7 +112,6 @@ def bench_batched(count, steps, seed=4, target_input_bytes=None):
     t_pc = timed(plain_c, steps)
     ref_streams = [s.clone() for s in streams]
     t_pd = timed(plain_d, steps)
-    ref_outs_sum = None
     plain_cg = batch.BatchPlan(cjobs, {"plain_launches": 1, "use_graph": 1})
     t_pcg = timed(plain_cg, steps)
     plan_c = batch.BatchPlan(cjobs)
@@ -125,7 +128,6 @@ def bench_batched(count, steps, seed=4, target_input_bytes=None):
         r = frsz_amd.decompress(s, x.numel(), x.dtype, b, B)
         same_d &= torch.equal(r.view(torch.uint8), o.view(torch.uint8))
     torch.cuda.synchronize()
-    del ref_outs_sum
     res.update({
         "batched_compress_ms": round(t_bc * 1e3, 3), "plain_compress_ms": round(t_pc * 1e3, 3),
         "plain_graph_compress_ms": round(t_pcg * 1e3, 3),
@@ -211,6 +213,107 @@ def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
     del x, y
     torch.cuda.empty_cache()
     return out
+
+
+def bench_small(reps=200, bits=16, block=32):
+    """The reference's own CPU-sized case: float32 x_i = sin(i/500) +
+    0.3 sin(i/37) + N(0, 1e-3), 2^20 values (4 MiB), bits 16, B 32.  At this
+    size a launch is bound by dispatch, not bandwidth: reported are the
+    average over `reps` back-to-back launches from Python, the same launches
+    replayed from a captured graph (20 per graph, no Python between them;
+    every replay's output is checked against the eager one), the median
+    single-launch time (synchronised before and after, event-timed) and the
+    host-memory call (pageable in, pageable out, wall clock), the plugin's
+    meaning.  Returns
+    the result and (values, GPU stream, GPU decode) for the checker."""
+    import numpy as np
+    import torch
+
+    import frsz_amd
+    from frsz_amd import _capi
+    from frsz_amd.workloads import smooth_signal
+
+    n = 1 << 20
+    x_host = smooth_signal(n, seed=1, dtype=np.float32)
+    x = torch.from_numpy(x_host).cuda()
+    size = frsz_amd.compressed_size(torch.float32, bits, block, n)
+    stream = torch.empty(size, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    alg = n * 4 + size
+
+    def comp():
+        frsz_amd.compress_into(x, stream, bits, block)
+
+    def decomp():
+        frsz_amd.decompress_into(stream, y, bits, block)
+
+    res = {"workload": f"float32 smooth signal, {n} values, bits {bits}, B {block}", "n": n, "stream_bytes": size}
+    for name, fn in (("compress", comp), ("decompress", decomp)):
+        for _ in range(20):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        t_b2b = e0.elapsed_time(e1) / reps * 1e-3
+        singles = []
+        for _ in range(50):
+            torch.cuda.synchronize()
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            singles.append(e0.elapsed_time(e1) * 1e-3)
+        t_one = float(np.median(singles))
+        res[f"{name}_us_back_to_back"] = round(t_b2b * 1e6, 2)
+        res[f"{name}_us_single"] = round(t_one * 1e6, 2)
+        res[f"{name}_gbs_back_to_back"] = round(alg / t_b2b / 1e9, 1)
+        # graph replays: the launches without Python dispatch between them
+        out = stream if name == "compress" else y
+        fn()
+        torch.cuda.synchronize()
+        want = out.clone()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(20):
+                fn()
+        same = True
+        best = 1e9
+        for _ in range(5):
+            out.zero_()
+            torch.cuda.synchronize()
+            e0.record()
+            graph.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3 / 20)
+            same &= bool(torch.equal(out, want))
+        del graph
+        res[f"{name}_us_graph"] = round(best * 1e6, 2)
+        res[f"{name}_gbs_graph"] = round(alg / best / 1e9, 1)
+        res[f"{name}_graph_replays_exact"] = same
+    comp()
+    decomp()
+    torch.cuda.synchronize()
+    s_host = np.empty(size, np.uint8)
+    y_host = np.empty_like(x_host)
+    for name, call in (("compress", lambda: _capi.lib.frsz2_compress_host(0, bits, block, x_host.ctypes.data, n,
+                                                                          s_host.ctypes.data, size)),
+                       ("decompress", lambda: _capi.lib.frsz2_decompress_host(0, bits, block, s_host.ctypes.data,
+                                                                              size, n, y_host.ctypes.data))):
+        ts = []
+        for _ in range(21):
+            t0 = time.perf_counter()
+            _capi.check(call())
+            ts.append(time.perf_counter() - t0)
+        res[f"host_{name}_us"] = round(float(np.median(ts[1:])) * 1e6, 1)
+    gpu_s, gpu_y = stream.cpu().numpy(), y.cpu().numpy()
+    res["host_path_same_bytes"] = bool(np.array_equal(s_host, gpu_s) and
+                                       np.array_equal(y_host.view(np.uint32), gpu_y.view(np.uint32)))
+    return res, (x_host, gpu_s, gpu_y, bits, block)
 
 
 def bench_host(x_host, bits, block, want_stream, want_y, reps=3):
@@ -421,13 +524,29 @@ def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_opti
             "pool_peak_gib": round(stats["pool_peak"] / 2**30, 3), "max_queue": stats["max_queue"]}
 
 
-def cpu_baseline(x_host, bits, block, sample):
-    """Time the reference CPU codec (or the C restatement when the reference
-    build is absent) on the first `sample` values; best of 2 passes."""
+def cpu_leg(checks, x_host, bits, block, sample, small_values, time_cpu=True):
+    """The CPU leg, the one part of this file that touches the oracle.
+
+    checks: {name: (values, GPU stream, GPU decode, bits, block)}; each is
+    compared byte-for-byte with the oracle (the multithreaded C restatement),
+    giving {name: bool}.  With time_cpu, the reference CPU codec (or the
+    restatement when the reference build is absent) is timed on the first
+    `sample` values of the field (best of 2 passes) and on the whole small
+    case (best of 5), giving the cpu_baseline object (else None)."""
     import numpy as np
 
     from oracle.oracle import Port, Reference, compressed_bytes, reference_available
 
+    port = Port().with_threads(0)
+    verdicts = {}
+    for name, (vals, got_s, got_y, b, blk) in checks.items():
+        want = port.compress(vals, b, blk)
+        ui = np.uint32 if vals.dtype == np.float32 else np.uint64
+        verdicts[name] = bool(np.array_equal(got_s, want) and
+                              np.array_equal(got_y.view(ui), port.decompress(want, vals.size, vals.dtype, b,
+                                                                              blk).view(ui)))
+    if not time_cpu:
+        return verdicts, None
     xs = np.ascontiguousarray(x_host[:sample])
     kind = "reference" if reference_available() else "port"
     codec = Reference() if kind == "reference" else Port()
@@ -455,7 +574,6 @@ def cpu_baseline(x_host, bits, block, sample):
         "nproc": os.cpu_count(),
     }
     # the block-parallel restatement on all host cores (same bytes, OpenMP)
-    port = Port().with_threads(0)
     t0 = time.perf_counter()
     s = port.compress(xs, bits, block)
     t1 = time.perf_counter()
@@ -463,7 +581,21 @@ def cpu_baseline(x_host, bits, block, sample):
     t2 = time.perf_counter()
     res["port_all_cores"] = {"threads": port.max_threads(), "value": round(2 * nbytes / (t2 - t0) / 1e9, 4),
                              "unit": "GB/s"}
-    return res
+    if small_values is not None:
+        # the small case whole, single-threaded (the reference's own test size)
+        sb, sblk = 16, 32
+        sc, sd = [], []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            s = codec.compress(small_values, sb, sblk)
+            t1 = time.perf_counter()
+            codec.decompress(s, small_values.size, small_values.dtype, sb, sblk)
+            t2 = time.perf_counter()
+            sc.append(t1 - t0)
+            sd.append(t2 - t1)
+        res["small_case"] = {"values": int(small_values.size), "bits": sb, "block": sblk, "kind": kind,
+                             "compress_us": round(min(sc) * 1e6, 1), "decompress_us": round(min(sd) * 1e6, 1)}
+    return verdicts, res
 
 
 def profiled_traffic(op, side, bits, block):
@@ -497,7 +629,6 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported (the codec "
                                    "shards by exponent-block ranges, see the design notes)"}))
         sys.exit(2)
-    import numpy as np
     import torch
 
     import frsz_amd
@@ -549,19 +680,9 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = 2 * alg / (wall / args.steps) / 1e9
 
-    verified = None
-    x_host = None
-    if not args.no_verify or not args.no_cpu:
-        x_host = x.cpu().numpy()
-    if not args.no_verify:
-        from oracle.oracle import Port
-
-        port = Port().with_threads(0)
-        want = port.compress(x_host, bits, block)
-        ok_s = np.array_equal(stream.cpu().numpy(), want)
-        ok_y = np.array_equal(y.cpu().numpy().view(np.uint64),
-                              port.decompress(want, n, np.float64, bits, block).view(np.uint64))
-        verified = bool(ok_s and ok_y)
+    x_host = s_host = y_host = None
+    if not args.no_verify or not args.no_cpu or not args.no_host:
+        x_host, s_host, y_host = x.cpu().numpy(), stream.cpu().numpy(), y.cpu().numpy()
 
     dom = "compress" if sum(tc) >= sum(td) else "decompress"
     dom_t = (sum(tc) if dom == "compress" else sum(td)) / args.steps
@@ -588,7 +709,7 @@ def main():
         "decompress_pct_hbm": round(100 * decomp_gbs / HBM_PEAK_GBS, 2),
         "compress_us": round(sum(tc) / len(tc) * 1e6, 2),
         "decompress_us": round(sum(td) / len(td) * 1e6, 2),
-        "bit_exact": verified,
+        "bit_exact": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(dom, side, bits, block), "algorithmic_bytes": alg},
@@ -612,11 +733,13 @@ def main():
         del a, b
     if not args.no_host:
         print("bench: host-resident path", file=sys.stderr, flush=True)
-        if x_host is None:
-            x_host = x.cpu().numpy()
-        line["host_path"] = bench_host(x_host, bits, block, stream.cpu().numpy(), y.cpu().numpy())
+        line["host_path"] = bench_host(x_host, bits, block, s_host, y_host)
     del x, y, stream
     torch.cuda.empty_cache()
+    small_check = None
+    if not args.no_small:
+        print("bench: small case", file=sys.stderr, flush=True)
+        line["small_case"], small_check = bench_small()
     if not args.no_batched:
         print("bench: batched workload", file=sys.stderr, flush=True)
         line["batched"] = bench_batched(args.batched_arrays, args.batched_steps)
@@ -627,9 +750,21 @@ def main():
         line["edge"] = bench_edge()
     if not args.no_serve:
         line["serve"] = bench_serve(args.serve_seconds, args.serve_rate, args.serve_budget_gib)
-    if not args.no_cpu:
-        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(x_host, bits, block, min(args.cpu_sample, n))
+    if not args.no_verify or not args.no_cpu:
+        print("bench: CPU leg (checker and CPU baseline)", file=sys.stderr, flush=True)
+        checks = {}
+        if not args.no_verify:
+            checks["field"] = (x_host, s_host, y_host, bits, block)
+            if small_check is not None:
+                checks["small"] = small_check
+        verdicts, cpu = cpu_leg(checks, x_host, bits, block, min(args.cpu_sample, n),
+                                None if small_check is None else small_check[0], time_cpu=not args.no_cpu)
+        if "field" in verdicts:
+            line["bit_exact"] = verdicts["field"]
+        if "small" in verdicts:
+            line["small_case"]["bit_exact"] = verdicts["small"]
+        if cpu is not None:
+            line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 
