@@ -5,9 +5,9 @@
 // The jobs' tiles form one global index space (job j owns tiles
 // [tile0_j, tile0_j + ntiles_j)).  A persistent grid of workgroups claims
 // tiles from a device counter (`chunk` tiles per atomic claim; chunk 0 =
-// static round-robin, no counter), finds the tile's job by binary search over
-// tile0 (starting from the previous job), and runs the same tile code as the
-// plain kernels for the job's class (direction, dtype, storage width U,
+// static round-robin, no counter), reads the tile's job from the plan's tile
+// table (or, without one, finds it by binary search over tile0), and runs the
+// same tile code as the plain kernels for the job's class (direction, dtype, storage width U,
 // packed or not).  Jobs the tile path cannot take (B not a power of two or
 // larger than a tile, unaligned pointers) run as plain one-thread-per-block
 // launches inside the same graph.  The counter is reset by a one-thread
@@ -291,6 +291,35 @@ batch_loop(const bjob* __restrict__ jobs, int njobs, std::uint64_t total_tiles, 
   __shared__ std::uint32_t lds[batch_lds_dwords<SHAPE>()];
   __shared__ std::uint32_t wred[WG / tile::WAVE];
   __shared__ unsigned long long claim;
+  if (chunk == 0 && jtab) {
+    // Round-robin with the job lookups run ahead: the descriptor of the next
+    // tile and the table entry of the one after it are loaded before the
+    // current tile runs, so their latency passes under its work instead of
+    // between tiles.  Every value loaded here is workgroup-uniform.
+    // (Past the last tile the lookups read the last tile's entry, so every
+    // load is unconditional and in bounds.)
+    const std::uint64_t G = gridDim.x;
+    const std::uint64_t last = total_tiles - 1;
+    std::uint64_t g = blockIdx.x;
+    if (g >= total_tiles) {
+      return;
+    }
+    bjob J = jobs[jtab[g]];
+    std::uint32_t jn = jtab[g + G < last ? g + G : last];
+    for (;;) {
+      const std::uint64_t gn = g + G;
+      const bjob Jn = jobs[jn];
+      const std::uint32_t jnn = jtab[gn + G < last ? gn + G : last];
+      run_tile<SHAPE, OPS>(J, g - J.tile0, lds, wred);
+      if (gn >= total_tiles) {
+        break;
+      }
+      g = gn;
+      J = Jn;
+      jn = jnn;
+    }
+    return;
+  }
   int j = 0;
   std::uint64_t next = blockIdx.x;
   for (;;) {
