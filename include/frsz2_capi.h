@@ -4,10 +4,12 @@
  * Plain pointers and sizes only.  Device pointers are HIP device memory on
  * the current device; `stream` is a hipStream_t passed as void* (NULL = the
  * default stream).  Every call is stream-ordered and asynchronous unless its
- * comment says otherwise; the library keeps no global state besides a
- * thread-local last-error string, so calls on different streams or threads
- * may run concurrently (the reference plugin declares
- * pressio:thread_safe = multiple, src/frsz2.cc:40).
+ * comment says otherwise.  The state shared between calls is the tuning
+ * table (frsz2_tune_*; every variant it can select writes the same bytes)
+ * and a pool of host staging pipelines that each host-memory call leases for
+ * its duration; the last-error string is per thread.  Calls on different
+ * streams or threads may therefore run concurrently (the reference plugin
+ * declares pressio:thread_safe = multiple, src/frsz2.cc:40).
  *
  * dtype: FRSZ2_FLOAT32 or FRSZ2_FLOAT64.
  * bits:  4..32 for float32, 4..64 for float64 (src/frsz2.cc:86-101 lists the
@@ -21,8 +23,8 @@
  * reference's message text "Unsupported number of bits: N" /
  * "Unsupported exponent block size: N" (include/frsz2.hpp:1300, :1326).
  * Codes 3..6 are new: capacity too small, bad pointer or alignment, HIP
- * runtime error, malformed container header.  frsz2_last_error() returns the message of the calling
- * thread's last failure.
+ * runtime error, malformed container header.  frsz2_last_error() returns
+ * the message of the calling thread's last failure.
  */
 #ifndef FRSZ2_CAPI_H
 #define FRSZ2_CAPI_H
