@@ -1,13 +1,12 @@
-"""In-tree build of the native parts (hipcc for gfx950, gcc / g++ for the
-host-only pieces).
+"""In-tree build of the codec library, frsz_amd/lib/libfrsz2_hip.so (hipcc
+for gfx950).
 
-    python frsz_amd/build.py          # everything below (run as a script: importing
-                                      # the package would load the old library)
-    build_library()                   # frsz_amd/lib/libfrsz2_hip.so
-    build_test_helpers()              # tests/native/*.so and the C++ API test
-    build_oracle()                    # oracle/libfrsz2_oracle.so (+ oracle/_ref)
+    python frsz_amd/build.py    # run as a script: importing the package would
+                                # load the library that is about to be rebuilt
 
-Every HIP source is compiled with the packed-fp32 instructions disabled.
+Every HIP source is compiled with the packed-fp32 instructions disabled.  The
+test helpers (tests/native/build.py) and the oracle (oracle/Makefile) have
+build recipes of their own.
 """
 from __future__ import annotations
 
@@ -77,36 +76,6 @@ def build_library() -> str:
     return LIB_PATH
 
 
-def build_test_helpers() -> None:
-    """Host build of the device value arithmetic (tests/native) and the C++
-    header-API test program (tests/native/test_header_api)."""
-    nat = os.path.join(ROOT, "tests", "native")
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
-          f"-I{INCLUDE}", "-o", os.path.join(nat, "libdevmath_host.so"), os.path.join(nat, "devmath_host.cpp")])
-    api_src = os.path.join(nat, "test_header_api.cpp")
-    if os.path.exists(api_src):
-        _run([hipcc(), f"--offload-arch={ARCH}", "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang",
-              "-packed-fp32-ops", f"-I{INCLUDE}", "-o", os.path.join(nat, "test_header_api"), api_src,
-              f"-L{LIBDIR}", "-lfrsz2_hip", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
-
-
-def build_oracle(with_reference: bool | None = None) -> None:
-    """The oracle's C restatement, and the reference CPU codec when the
-    reference checkout is present (oracle/Makefile)."""
-    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
-    ref_root = os.environ.get("FRSZ2_REFERENCE_ROOT", "/root/reference")
-    if with_reference is None:
-        with_reference = os.path.exists(os.path.join(ref_root, "include", "frsz2.hpp"))
-    if with_reference:
-        _run(["make", "-s", "-j6", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF_ROOT={ref_root}"])
-
-
-def build_all() -> None:
-    build_library()
-    build_test_helpers()
-    build_oracle()
-
-
 if __name__ == "__main__":
-    build_all()
+    build_library()
     print("built", LIB_PATH)

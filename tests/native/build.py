@@ -1,0 +1,45 @@
+"""Build of the native test helpers (test infrastructure):
+
+* libdevmath_host.so: the device value arithmetic of include/frsz2_device.hpp
+  compiled for the host, which tests/test_devmath.py checks against the oracle;
+* test_header_api: the C++ header-API program (include/frsz2.hpp) that
+  tests/test_gpu_header_api.py runs, linked against frsz_amd/lib/libfrsz2_hip.so
+  (build that first).
+
+    python tests/native/build.py
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+INCLUDE = os.path.join(ROOT, "include")
+LIBDIR = os.path.join(ROOT, "frsz_amd", "lib")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        # hipcc's host pass does not know the packed-fp32 feature; that warning is harmless
+        sys.stderr.write("\n".join(ln for ln in res.stdout.splitlines() if "packed-fp32-ops" not in ln) + "\n")
+        raise RuntimeError(f"command failed ({res.returncode}): {' '.join(cmd)}")
+
+
+def build_test_helpers() -> None:
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+          f"-I{INCLUDE}", "-o", os.path.join(HERE, "libdevmath_host.so"), os.path.join(HERE, "devmath_host.cpp")])
+    hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
+    _run([hipcc, "--offload-arch=gfx950", "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", f"-I{INCLUDE}", "-o", os.path.join(HERE, "test_header_api"),
+          os.path.join(HERE, "test_header_api.cpp"), f"-L{LIBDIR}", "-lfrsz2_hip", f"-Wl,-rpath,{LIBDIR}",
+          "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
+
+
+if __name__ == "__main__":
+    build_test_helpers()
+    print("built test helpers in", HERE)
