@@ -3,9 +3,9 @@
  * (robertu94/frsz, include/frsz2.hpp).
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load this library, and only as the checker
- * (or, for bench.py, as the timed "port" CPU baseline).  The product never
- * links it.
+ * CPU leg of bench.py (cpu_leg, which yields cpu_baseline) load this
+ * library, and only as the checker (or, for bench.py, also as the timed
+ * "port" CPU baseline).  The product never links it.
  *
  * Parity pins (what checks each function; see tests/test_oracle.py):
  *   fo_exponent_* / fo_significand_* / fo_sign_*  -- the reference's known-

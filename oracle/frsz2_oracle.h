@@ -4,7 +4,8 @@
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (frsz_amd/, include/)
  * links, loads or calls this code.  It is used by tests/ as the bit-exact
  * checker, by __graft_entry__.smoke() as the checker of one small GPU call,
- * and by bench.py's cpu_baseline leg as the "port" CPU baseline.
+ * and by bench.py's CPU leg (cpu_leg) as the checker of the benchmark's GPU
+ * outputs and the "port" CPU baseline.
  *
  * Every function follows a reference function in include/frsz2.hpp of
  * robertu94/frsz; the citation is given at each definition in

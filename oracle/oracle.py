@@ -1,8 +1,9 @@
 """ctypes access to the FRSZ2 oracle libraries.
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` import this module, and only as the
-checker (or the timed CPU baseline).  The product package never imports it.
+CPU leg of ``bench.py`` (``cpu_leg``, which yields ``cpu_baseline``) import
+this module, and only as the checker (or the timed CPU baseline).  The
+product package never imports it.
 
 Two libraries are exposed with the same numpy-level API:
 
