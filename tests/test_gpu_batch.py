@@ -111,6 +111,39 @@ def test_few_jobs(fz, count, cfg):
     dplan.close()
 
 
+@pytest.mark.parametrize("shape", [0, 2])
+@pytest.mark.parametrize("grid", [1, 5, 6, 11, 12, 13, 64])
+def test_round_robin_grid_edges(fz, shape, grid):
+    """Static round-robin (no claims) over a tile table: twelve float32 jobs of
+    one tile each (more than fit in the kernel arguments), with grids that
+    take every tile in one step, leave a remainder, end exactly on a full
+    round, or exceed the tile count.  The job lookups run one and two tiles
+    ahead of the tile in hand, so these are the cases where they reach past
+    the last tile."""
+    fz_mod, batch = fz
+    from frsz_amd.workloads import smooth_signal_torch
+
+    specs = []
+    for i in range(12):
+        x = smooth_signal_torch(1000 - 37 * i, seed=500 + i, dtype=torch.float32)
+        specs.append((x, (16, 12, 21, 32)[i % 4], 32))
+    ref_s, ref_v = plain_outputs(fz_mod, specs)
+    cfg = {"tile_shape": shape, "workgroups": grid, "tiles_per_claim": 0, "use_graph": 1}
+    outs = [torch.full_like(s, 0x5A) for s in ref_s]
+    plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
+    vals = [torch.full_like(x, float("nan")) for x, _, _ in specs]
+    dplan = batch.BatchPlan([batch.decompress_job(s, v, b, B) for (x, b, B), s, v in zip(specs, ref_s, vals)], cfg)
+    plan.launch()
+    dplan.launch()
+    torch.cuda.synchronize()
+    for i, (o, r) in enumerate(zip(outs, ref_s)):
+        assert torch.equal(o, r), f"grid {grid} shape {shape} job {i}"
+    for i, (v, r) in enumerate(zip(vals, ref_v)):
+        assert torch.equal(v.view(torch.uint8), r.view(torch.uint8)), f"grid {grid} shape {shape} decompress job {i}"
+    plan.close()
+    dplan.close()
+
+
 def test_graph_replays_with_eager_work_between(fz):
     fz_mod, batch = fz
     specs = make_arrays(2, count=30)
