@@ -6,12 +6,12 @@
 // [tile0_j, tile0_j + ntiles_j)).  A persistent grid of workgroups claims
 // tiles from a device counter (`chunk` tiles per atomic claim; chunk 0 =
 // static round-robin, no counter), reads the tile's job from the plan's tile
-// table (or, without one, finds it by binary search over tile0), and runs the
-// same tile code as the plain kernels for the job's class (direction, dtype, storage width U,
-// packed or not).  Jobs the tile path cannot take (B not a power of two or
-// larger than a tile, unaligned pointers) run as plain one-thread-per-block
-// launches inside the same graph.  The counter is reset by a one-thread
-// kernel captured at the head of the graph.
+// table (or, without one, finds it by binary search over tile0), and runs
+// the same tile code as the plain kernels for the job's class (direction,
+// dtype, storage width U, packed or not).  Jobs the tile path cannot take (B
+// not a power of two or larger than a tile, unaligned pointers) run as plain
+// launches of the generic kernels inside the same graph.  The counter is
+// reset by a one-thread kernel captured at the head of the graph.
 //
 // Tile shapes are per class and come in three settings (narrow, balanced,
 // wide; see class_vpt).  The autotuner times candidate (workgroups, tile

@@ -2,8 +2,10 @@
 //
 //   compress    k_compress_tile<T,U,VPT,PACKED>   one 256-thread workgroup per
 //               tile of 256*VPT values (power-of-two B <= tile), XCD-aware
-//               tile order; k_compress_block for any other B (one thread per
-//               exponent block) and for pointers that are not 16-byte aligned.
+//               tile order; for any other B and for pointers that are not
+//               16-byte aligned, k_compress_block (one thread per block, B up
+//               to 16) or k_block_exponents (block maxima) then
+//               k_encode_words (one thread per payload word).
 //   decompress  k_decompress_tile<...> / k_decompress_elem (one thread per value)
 //   gather      k_gather (one thread per index)
 //
@@ -66,9 +68,13 @@ __global__ __launch_bounds__(WGS) void k_decompress_tile(const std::uint8_t* __r
   tile::decompress_tile<T, U, VPT, PACKED, NT, STAGE, WGS>(in, out, p, t, smem, smem + LD);
 }
 
-// One thread per exponent block, any B (encode_block: each payload word is
-// built from the codes that overlap it, all CB bytes of the block written).
-// The first 256 threads clear the zpad exponent-pad bytes after slot nb.
+// Generic compress, blocks of up to GENERIC_SMALL_B values: one thread per
+// exponent block (encode_block: each payload word is built from the codes
+// that overlap it, all CB bytes of the block written).  A thread's values
+// are consecutive, so a wave's loads stay within a few cache lines.  The
+// first 256 threads clear the zpad exponent-pad bytes after slot nb.
+constexpr std::uint64_t GENERIC_SMALL_B = 16;
+
 template<typename T, int U>
 __global__ __launch_bounds__(WG) void k_compress_block(const T* __restrict__ in,
                                                        std::uint8_t* __restrict__ out,
@@ -88,6 +94,106 @@ __global__ __launch_bounds__(WG) void k_compress_block(const T* __restrict__ in,
   const std::uint64_t first = gid * L.B;
   const std::uint64_t len = (L.n - first) < L.B ? (L.n - first) : L.B;
   encode_block<T, U>(out, L, gid, in + first, len);
+}
+
+// Generic compress, larger blocks (any B, any alignment), in two passes that
+// read the input with consecutive lanes on consecutive values (one thread
+// per block would read a block's values serially, lanes B values apart).
+//
+// Pass 1: the exponent of every block (compress_cpu_block's block maximum,
+// include/frsz2.hpp:1121-1128).  A group of G = 2^lg lanes (G divides the
+// wave) takes one block: lane l reads values l, l + G, ... of it, and the
+// key maximum is a butterfly over the group.  The first 256 threads clear the
+// zpad exponent-pad bytes after slot nb.
+template<typename T, int U>
+__global__ __launch_bounds__(WG) void k_block_exponents(const T* __restrict__ in,
+                                                        std::uint8_t* __restrict__ out,
+                                                        layout L,
+                                                        std::uint32_t zpad,
+                                                        int lg)
+{
+  using uw = typename fmt<T>::uw;
+  const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (gid < 256) {
+    std::uint8_t* q = out + L.nb * static_cast<std::uint64_t>(U / 8);
+    for (std::uint32_t i = static_cast<std::uint32_t>(gid); i < zpad; i += 256) {
+      q[i] = 0;
+    }
+  }
+  const int G = 1 << lg;
+  const std::uint64_t k = gid >> lg;
+  const std::uint64_t lane = gid & static_cast<std::uint64_t>(G - 1);
+  std::uint32_t m = 0;
+  if (k < L.nb) {
+    const std::uint64_t first = k * L.B;
+    const std::uint64_t count = (L.n - first) < L.B ? (L.n - first) : L.B;
+    for (std::uint64_t j = lane; j < count; j += static_cast<std::uint64_t>(G)) {
+      const std::uint32_t key = max_key<T>(__builtin_bit_cast(uw, in[first + j]));
+      m = m > key ? m : key;
+    }
+  }
+  // every lane of the wave takes part in the exchange (none has returned)
+  for (int o = G >> 1; o > 0; o >>= 1) {
+    const std::uint32_t other = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(m), o, G));
+    m = m > other ? m : other;
+  }
+  if (k < L.nb && lane == 0) {
+    store_block_exponent<U>(out, k, block_exponent_from_key<T, U>(m));
+  }
+}
+
+// Pass 2: one thread per payload word, built from the codes that overlap it
+// (as encode_block does: codes past the block's count are zero), with the
+// block exponents of pass 1.  wpb = payload words per block.  Offsets within
+// a block fit 32 bits (B <= 2^24, bits <= 64), and so does the word index
+// when the stream has fewer than 2^32 payload words (narrow = true), which
+// keeps the divisions 32-bit.
+template<typename T, int U>
+__global__ __launch_bounds__(WG) void k_encode_words(const T* __restrict__ in,
+                                                     std::uint8_t* __restrict__ out,
+                                                     layout L,
+                                                     std::uint32_t wpb,
+                                                     bool narrow)
+{
+  using uw = typename fmt<T>::uw;
+  using wt = typename word<U>::type;
+  const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (gid >= L.nb * wpb) {
+    return;
+  }
+  std::uint64_t k;
+  std::uint32_t w;
+  if (narrow) {
+    const std::uint32_t g32 = static_cast<std::uint32_t>(gid);
+    k = g32 / wpb;
+    w = g32 - static_cast<std::uint32_t>(k) * wpb;
+  } else {
+    k = gid / wpb;
+    w = static_cast<std::uint32_t>(gid - k * wpb);
+  }
+  const std::uint64_t first = k * L.B;
+  const std::uint32_t count = static_cast<std::uint32_t>((L.n - first) < L.B ? (L.n - first) : L.B);
+  const int E = load_block_exponent<U>(out, k);
+  const std::uint32_t bits = static_cast<std::uint32_t>(L.bits);
+  const T* v = in + first;
+  std::uint64_t acc = 0;
+  if (bits == U) {
+    // one code per word
+    if (w < count) {
+      acc = static_cast<std::uint64_t>(encode_value<T>(__builtin_bit_cast(uw, v[w]), E, L.bits));
+    }
+  } else {
+    const std::uint32_t lo = w * U;
+    const std::uint32_t j0 = lo / bits;
+    std::uint32_t j1 = (lo + U + bits - 1) / bits;
+    j1 = j1 < count ? j1 : count;
+    for (std::uint32_t j = j0; j < j1; ++j) {
+      const std::uint64_t c = static_cast<std::uint64_t>(encode_value<T>(__builtin_bit_cast(uw, v[j]), E, L.bits));
+      const int sh = static_cast<int>(j * bits) - static_cast<int>(lo);
+      acc |= sh >= 0 ? (c << sh) : (c >> (-sh));
+    }
+  }
+  reinterpret_cast<wt*>(out + L.R + k * L.CB)[w] = static_cast<wt>(acc);
 }
 
 template<typename T, int U>
@@ -187,9 +293,28 @@ template<typename T, int U>
 hipError_t
 launch_compress_generic(const void* in, void* out, const layout& L, std::uint32_t zpad, hipStream_t s)
 {
-  std::uint64_t threads = L.nb < 256 ? 256 : L.nb;
-  hipLaunchKernelGGL((k_compress_block<T, U>), dim3(grid_for(threads)), dim3(WG), 0, s, static_cast<const T*>(in),
-                     static_cast<std::uint8_t*>(out), L, zpad);
+  if (L.B <= GENERIC_SMALL_B) {
+    const std::uint64_t threads = L.nb < 256 ? 256 : L.nb;
+    hipLaunchKernelGGL((k_compress_block<T, U>), dim3(grid_for(threads)), dim3(WG), 0, s, static_cast<const T*>(in),
+                       static_cast<std::uint8_t*>(out), L, zpad);
+    return hipGetLastError();
+  }
+  // G lanes per block: the power of two at or above B, at most a wave
+  int lg = 0;
+  while (lg < 6 && (std::uint64_t{ 1 } << lg) < L.B) {
+    ++lg;
+  }
+  const std::uint64_t t1 = L.nb << lg;
+  hipLaunchKernelGGL((k_block_exponents<T, U>), dim3(grid_for(t1 < 256 ? 256 : t1)), dim3(WG), 0, s,
+                     static_cast<const T*>(in), static_cast<std::uint8_t*>(out), L, zpad, lg);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess || L.nb == 0) {
+    return e;
+  }
+  const std::uint64_t wpb = L.CB / static_cast<std::uint64_t>(U / 8);
+  const std::uint64_t words = L.nb * wpb;
+  hipLaunchKernelGGL((k_encode_words<T, U>), dim3(grid_for(words)), dim3(WG), 0, s, static_cast<const T*>(in),
+                     static_cast<std::uint8_t*>(out), L, static_cast<std::uint32_t>(wpb), words < (std::uint64_t{ 1 } << 32));
   return hipGetLastError();
 }
 
@@ -220,7 +345,7 @@ launch_gather(const void* in, const layout& L, const std::uint64_t* idx, std::ui
 // Values per thread actually launched: the tuned one, raised to the smallest
 // supported one whose tile (WGS x VPT values) holds a whole exponent block.
 // 2 values per thread exists for float64 with U >= 16 only (one 16-byte load
-// per lane).  0 = no tile holds the block (the one-thread-per-block kernels).
+// per lane).  0 = no tile holds the block (the generic kernels).
 template<typename T, int U>
 int
 effective_vpt(int vpt, std::uint64_t B, int wgs)
@@ -235,20 +360,35 @@ effective_vpt(int vpt, std::uint64_t B, int wgs)
 }
 
 // Tiled launch with the variant of the tuning table (values per thread,
-// non-temporal hint, tile order, staged decompress stores, workgroup size);
-// B beyond the largest tile, or unaligned pointers, take the
-// one-thread-per-block kernels.  Workgroups of 64 threads are built with the
-// non-temporal hint only.
+// non-temporal hint, tile order, staged decompress stores, workgroup size).
+// A block too large for a 64-thread tile takes the 256-thread one; B beyond
+// the largest tile, or unaligned pointers, take the generic kernels.
+// Workgroups of 64 threads are built with the non-temporal hint only.
+template<typename T, int U>
+int
+tiled_vpt(bool tiled, int vpt, std::uint64_t B, bool& small)
+{
+  if (!tiled) {
+    return 0;
+  }
+  int v = effective_vpt<T, U>(vpt, B, small ? 64 : WG);
+  if (v == 0 && small) {
+    small = false;
+    v = effective_vpt<T, U>(vpt, B, WG);
+  }
+  return v;
+}
+
 template<typename T, int U>
 hipError_t
 run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
-  const int wgs = v.small ? 64 : WG;
-  int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B, wgs) : 0;
+  bool small = v.small != 0;
+  int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small);
   if (vpt > 0) {
 #define FRSZ2_CL(V, N, W) return launch_compress_tiled<T, U, V, N, W>(in, out, L, zpad, lb, v.linear, s)
-    if (v.small) {
+    if (small) {
       if constexpr (sizeof(T) == 8 && U >= 16) {
         if (vpt == 2) {
           FRSZ2_CL(2, true, 64);
@@ -287,11 +427,11 @@ hipError_t
 run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
-  const int wgs = v.small ? 64 : WG;
-  int vpt = tiled ? effective_vpt<T, U>(v.vpt, L.B, wgs) : 0;
+  bool small = v.small != 0;
+  int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small);
   if (vpt > 0) {
 #define FRSZ2_DL(V, N, S, W) return launch_decompress_tiled<T, U, V, N, S, W>(in, out, L, lb, v.linear, s)
-    if (v.small) {
+    if (small) {
       if constexpr (sizeof(T) == 8 && U >= 16) {
         if (vpt == 2) {
           if (v.stage) {

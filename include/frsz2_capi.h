@@ -15,8 +15,11 @@
  * bits:  4..32 for float32, 4..64 for float64 (src/frsz2.cc:86-101 lists the
  *        same ranges; include/frsz2.hpp:511-517 asserts 4 <= bits <= W).
  * block: the exponent block size B ("max_exp_block_size"), any value in
- *        1..FRSZ2_MAX_BLOCK.  Powers of two up to 1024 take the tiled
- *        kernels; other sizes take a slower one-thread-per-block kernel.
+ *        1..FRSZ2_MAX_BLOCK.  Powers of two up to 4096 take the tiled
+ *        kernels (with 16-byte aligned values and payload); other sizes
+ *        take slower generic kernels (compress: one thread per block up to
+ *        B = 16, else a block-maximum pass and one thread per payload word;
+ *        decompress and gather: one thread per value).
  *
  * Status codes follow the reference plugin: 0 ok, 1 unsupported dtype
  * (src/frsz2.cc:164, :181), 2 unsupported bits / block size with the

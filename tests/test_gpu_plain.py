@@ -91,11 +91,13 @@ def mixed_values(rng, n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 3, 5, 9, 100, 2048])
+@pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 3, 5, 9, 100, 2048, 24, 4096, 5000])
 def test_random_parity_all_bits(fz, port, dtype, B):
     """Every bit width 4..W against the oracle, sizes spanning several tiles
-    plus a ragged tail; power-of-two B <= 1024 take the tiled kernels, the
-    other sizes the one-thread-per-block kernel."""
+    plus a ragged tail.  Power-of-two B up to 4096 take the tiled kernels
+    (beyond a 64-thread tile, the 256-thread one); other sizes take the
+    generic compress: one thread per block up to B = 16, else the block-max
+    pass and one thread per payload word."""
     rng = np.random.default_rng(B * 7 + (dtype == np.float64))
     W = 32 if dtype == np.float32 else 64
     for bits in range(4, W + 1):
@@ -148,9 +150,12 @@ def test_no_write_outside_stream(fz):
         assert np.all(tail == 0xAB)
 
 
-def test_unaligned_pointers_take_generic_path(fz, port):
-    rng = np.random.default_rng(4)
-    n, bits, B = 10001, 16, 32
+@pytest.mark.parametrize("bits,B", [(16, 32), (12, 8), (21, 64)])
+def test_unaligned_pointers_take_generic_path(fz, port, bits, B):
+    """Values 4 bytes off 16-byte alignment: the generic kernels (B = 8
+    takes one thread per block, larger B the two passes)."""
+    rng = np.random.default_rng(4 + B)
+    n = 10001
     x = mixed_values(rng, n + 1, np.float32)
     xd = torch.from_numpy(x).cuda()[1:]  # 4-byte offset: not 16-byte aligned
     s = fz.compress(xd, bits, B)
