@@ -424,24 +424,26 @@ run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, boo
 
 template<typename T, int U>
 hipError_t
-run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+run_decompress(const void* in, void* out, const layout& L, bool tiled, bool strided, int lb, hipStream_t s)
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   bool small = v.small != 0;
   int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small);
+  // strided: the decode that stores single values (any value alignment)
+  const int stage = strided ? 0 : v.stage;
   if (vpt > 0) {
 #define FRSZ2_DL(V, N, S, W) return launch_decompress_tiled<T, U, V, N, S, W>(in, out, L, lb, v.linear, s)
     if (small) {
       if constexpr (sizeof(T) == 8 && U >= 16) {
         if (vpt == 2) {
-          if (v.stage) {
+          if (stage) {
             FRSZ2_DL(2, true, true, 64);
           }
           FRSZ2_DL(2, true, false, 64);
         }
       }
       vpt = vpt < 4 ? 4 : vpt;
-      switch (vpt * 2 + v.stage) {
+      switch (vpt * 2 + stage) {
         case 8: FRSZ2_DL(4, true, false, 64);
         case 9: FRSZ2_DL(4, true, true, 64);
         case 16: FRSZ2_DL(8, true, false, 64);
@@ -452,7 +454,7 @@ run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, h
     }
     if constexpr (sizeof(T) == 8 && U >= 16) {
       if (vpt == 2) {
-        switch (v.stage * 2 + v.nt) {
+        switch (stage * 2 + v.nt) {
           case 0: FRSZ2_DL(2, false, false, WG);
           case 1: FRSZ2_DL(2, true, false, WG);
           case 2: FRSZ2_DL(2, false, true, WG);
@@ -460,7 +462,7 @@ run_decompress(const void* in, void* out, const layout& L, bool tiled, int lb, h
         }
       }
     }
-    switch (vpt * 4 + v.stage * 2 + v.nt) {
+    switch (vpt * 4 + stage * 2 + v.nt) {
       case 16: FRSZ2_DL(4, false, false, WG);
       case 17: FRSZ2_DL(4, true, false, WG);
       case 18: FRSZ2_DL(4, false, true, WG);
@@ -518,9 +520,9 @@ struct compress_op
 template<typename T, int U>
 struct decompress_op
 {
-  static hipError_t run(const void* in, void* out, const layout& L, bool tiled, int lb, hipStream_t s)
+  static hipError_t run(const void* in, void* out, const layout& L, bool tiled, bool strided, int lb, hipStream_t s)
   {
-    return run_decompress<T, U>(in, out, L, tiled, lb, s);
+    return run_decompress<T, U>(in, out, L, tiled, strided, lb, s);
   }
 };
 
@@ -619,8 +621,17 @@ decompress_run(int dtype, const void* d_in, void* d_values, const layout& L, hip
     return FRSZ2_OK;
   }
   const int lb = ilog2_exact(L.B);
-  const bool tiled = lb >= 0 && aligned(d_values, 16) && aligned(static_cast<const std::uint8_t*>(d_in) + L.R, 16);
-  return hip_status(by_type<decompress_op>(dtype, L.U, d_in, d_values, L, tiled, lb, s), "decompress launch");
+  const bool vals16 = aligned(d_values, 16);
+  const bool pay16 = aligned(static_cast<const std::uint8_t*>(d_in) + L.R, 16);
+  // Values off 16-byte alignment, or an unpacked payload off it, still take
+  // the tiles through the strided decode: it stores single values and reads
+  // an unpacked payload word by word.  A packed payload is staged with
+  // 16-byte loads, so it needs the alignment.
+  const bool packed = L.bits < L.U;
+  const bool tiled = lb >= 0 && (pay16 || !packed);
+  const bool strided = !(vals16 && pay16);
+  return hip_status(by_type<decompress_op>(dtype, L.U, d_in, d_values, L, tiled, strided, lb, s),
+                    "decompress launch");
 }
 
 int

@@ -166,6 +166,33 @@ def test_unaligned_pointers_take_generic_path(fz, port, bits, B):
     assert np.array_equal(ubits(out.cpu().numpy()), ubits(port.decompress(want, n, np.float32, bits, B)))
 
 
+@pytest.mark.parametrize("dtype,bits,B", [(np.float32, 16, 32), (np.float32, 12, 32), (np.float64, 32, 64),
+                                          (np.float64, 21, 16)])
+def test_stream_and_values_off_16_byte_alignment(fz, port, dtype, bits, B):
+    """A stream 8 bytes off 16-byte alignment and values one element off it:
+    unpacked codes still take the tiles (strided decode, word loads), packed
+    ones the generic kernels; the bytes are the oracle's either way."""
+    rng = np.random.default_rng(bits * 31 + B)
+    n = 20011
+    x = mixed_values(rng, n, dtype)
+    want = port.compress(x, bits, B)
+    sbuf = torch.zeros(want.size + 8, dtype=torch.uint8, device="cuda")
+    s = sbuf[8:]
+    assert s.data_ptr() % 16 == 8
+    s.copy_(torch.from_numpy(want))
+    out = torch.empty(n + 1, dtype=tdt(dtype), device="cuda")[1:]
+    assert out.data_ptr() % 16 != 0
+    fz.decompress_into(s, out, bits, B)
+    assert np.array_equal(ubits(out.cpu().numpy()), ubits(port.decompress(want, n, dtype, bits, B)))
+    aligned = torch.empty(n, dtype=tdt(dtype), device="cuda")
+    fz.decompress_into(s, aligned, bits, B)
+    assert np.array_equal(ubits(aligned.cpu().numpy()), ubits(port.decompress(want, n, dtype, bits, B)))
+    xs = torch.from_numpy(x).cuda()
+    s.zero_()
+    fz.compress_into(xs, s, bits, B)
+    assert np.array_equal(s.cpu().numpy(), want)
+
+
 def test_gather_out_of_range_is_zero(fz):
     x = torch.ones(1000, dtype=torch.float32, device="cuda")
     s = fz.compress(x, 16, 32)
