@@ -34,7 +34,8 @@ using tile::WG;
 // WGS: threads per workgroup, 256 or 64 (one wave; the tile is then a
 // quarter as large and a workgroup's resources free as soon as its one wave
 // ends)
-template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS>
+// VEC = false: values aligned to their size only (loaded one by one)
+template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS, bool VEC = true>
 __global__ __launch_bounds__(WGS) void k_compress_tile(const typename fmt<T>::uw* __restrict__ in,
                                                        std::uint8_t* __restrict__ out,
                                                        tplan p,
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(WGS) void k_compress_tile(const typename fmt<T>::uw
   __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
   __shared__ std::uint32_t wred[WGS / tile::WAVE];
   const std::uint32_t t = linear ? blockIdx.x : tile::xcd_tile(blockIdx.x, ntiles);
-  tile::compress_tile<T, U, VPT, PACKED, NT, WGS>(in, out, p, t, lds, wred);
+  tile::compress_tile<T, U, VPT, PACKED, NT, WGS, VEC>(in, out, p, t, lds, wred);
   if (t == 0) {
     tile::zero_plan_pad<U>(out, p, static_cast<int>(threadIdx.x), WGS);
   }
@@ -244,7 +245,7 @@ grid_for(std::uint64_t work)
   return static_cast<unsigned>((work + WG - 1) / WG);
 }
 
-template<typename T, int U, int VPT, bool NT, int WGS>
+template<typename T, int U, int VPT, bool NT, int WGS, bool VEC = true>
 hipError_t
 launch_compress_tiled(const void* in, void* out, const layout& L, std::uint32_t zpad, int lb, int linear,
                       hipStream_t s)
@@ -258,10 +259,10 @@ launch_compress_tiled(const void* in, void* out, const layout& L, std::uint32_t 
   const auto* x = static_cast<const typename fmt<T>::uw*>(in);
   auto* o = static_cast<std::uint8_t*>(out);
   if (L.bits == U) {
-    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false, NT, WGS>), dim3(nt), dim3(WGS), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, false, NT, WGS, VEC>), dim3(nt), dim3(WGS), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt), linear);
   } else {
-    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true, NT, WGS>), dim3(nt), dim3(WGS), 0, s, x, o, p,
+    hipLaunchKernelGGL((k_compress_tile<T, U, VPT, true, NT, WGS, VEC>), dim3(nt), dim3(WGS), 0, s, x, o, p,
                        static_cast<std::uint32_t>(nt), linear);
   }
   return hipGetLastError();
@@ -381,8 +382,17 @@ tiled_vpt(bool tiled, int vpt, std::uint64_t B, bool& small)
 
 template<typename T, int U>
 hipError_t
-run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, int lb, hipStream_t s)
+run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, bool elementwise, int lb,
+             hipStream_t s)
 {
+  if (tiled && elementwise) {
+    // values aligned to their size only: the 4096-value tile that loads them
+    // one by one (it holds any power-of-two block up to 4096)
+    if (L.B <= static_cast<std::uint64_t>(WG) * 16) {
+      return launch_compress_tiled<T, U, 16, true, WG, false>(in, out, L, zpad, lb, 0, s);
+    }
+    return launch_compress_generic<T, U>(in, out, L, zpad, s);
+  }
   const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   bool small = v.small != 0;
   int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small);
@@ -510,10 +520,10 @@ by_type(int dtype, int U, A&&... a)
 template<typename T, int U>
 struct compress_op
 {
-  static hipError_t run(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled, int lb,
-                        hipStream_t s)
+  static hipError_t run(const void* in, void* out, const layout& L, std::uint32_t zpad, bool tiled,
+                        bool elementwise, int lb, hipStream_t s)
   {
-    return run_compress<T, U>(in, out, L, zpad, tiled, lb, s);
+    return run_compress<T, U>(in, out, L, zpad, tiled, elementwise, lb, s);
   }
 };
 
@@ -610,8 +620,12 @@ compress_run(int dtype, const void* d_values, void* d_out, const layout& L, std:
     return FRSZ2_OK;
   }
   const int lb = ilog2_exact(L.B);
-  const bool tiled = lb >= 0 && aligned(d_values, 16) && aligned(static_cast<const std::uint8_t*>(d_out) + L.R, 16);
-  return hip_status(by_type<compress_op>(dtype, L.U, d_values, d_out, L, zpad, tiled, lb, s), "compress launch");
+  // the tiles write the payload with 16-byte stores; values off 16-byte
+  // alignment are loaded one by one
+  const bool tiled = lb >= 0 && aligned(static_cast<const std::uint8_t*>(d_out) + L.R, 16);
+  const bool elementwise = !aligned(d_values, 16);
+  return hip_status(by_type<compress_op>(dtype, L.U, d_values, d_out, L, zpad, tiled, elementwise, lb, s),
+                    "compress launch");
 }
 
 int

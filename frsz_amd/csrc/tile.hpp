@@ -537,8 +537,10 @@ chunk_exponents(const std::uint32_t (&key)[NCH][VC],
 // Compress one tile.  A thread's VPT values are NCH chunks of VC = 16 bytes
 // of input (4 float32 / 2 float64), chunk c of thread t holding tile values
 // [(c*WGS + t)*VC, +VC), so every load instruction of a wave reads 1 KiB of
-// consecutive input and every code store writes consecutive bytes.
-template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS = WG>
+// consecutive input and every code store writes consecutive bytes.  VEC =
+// false loads the values one by one (input aligned to its element size
+// only; the payload must still be 16-byte aligned).
+template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS = WG, bool VEC = true>
 __device__ __forceinline__ void
 compress_tile(const typename fmt<T>::uw* __restrict__ in,
               std::uint8_t* __restrict__ out,
@@ -559,8 +561,13 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
     const std::uint64_t i0 = base + static_cast<std::uint64_t>(c * WGS + t) * VC;
-    if (i0 + VC <= p.n) {
+    if (VEC && i0 + VC <= p.n) {
       load_values<T, VC, NT>(in + i0, x[c]);
+    } else if (i0 + VC <= p.n) {
+#pragma unroll
+      for (int v = 0; v < VC; ++v) {
+        x[c][v] = ld<NT>(in + i0 + v);
+      }
     } else {
 #pragma unroll
       for (int v = 0; v < VC; ++v) {

@@ -16,8 +16,9 @@
  *        same ranges; include/frsz2.hpp:511-517 asserts 4 <= bits <= W).
  * block: the exponent block size B ("max_exp_block_size"), any value in
  *        1..FRSZ2_MAX_BLOCK.  Powers of two up to 4096 take the tiled
- *        kernels (compress: with 16-byte aligned values and payload;
- *        decompress: with a 16-byte aligned payload when bits < U); other
+ *        kernels (compress: with a 16-byte aligned payload; decompress:
+ *        with a 16-byte aligned payload when bits < U; values need only
+ *        their element alignment); other
  *        cases take slower generic kernels (compress: one thread per block
  *        up to B = 16, else a block-maximum pass and one thread per payload
  *        word; decompress and gather: one thread per value).

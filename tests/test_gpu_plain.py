@@ -150,10 +150,11 @@ def test_no_write_outside_stream(fz):
         assert np.all(tail == 0xAB)
 
 
-@pytest.mark.parametrize("bits,B", [(16, 32), (12, 8), (21, 64)])
+@pytest.mark.parametrize("bits,B", [(16, 32), (12, 8), (21, 64), (12, 4096), (16, 5000), (32, 1)])
 def test_unaligned_pointers_take_generic_path(fz, port, bits, B):
-    """Values 4 bytes off 16-byte alignment: the generic kernels (B = 8
-    takes one thread per block, larger B the two passes)."""
+    """Values 4 bytes off 16-byte alignment: compress loads them one by one
+    in the 4096-value tile (power-of-two B up to 4096) or takes the generic
+    kernels (B = 5000); decompress stores them through the strided decode."""
     rng = np.random.default_rng(4 + B)
     n = 10001
     x = mixed_values(rng, n + 1, np.float32)
