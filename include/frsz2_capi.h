@@ -18,10 +18,10 @@
  *        1..FRSZ2_MAX_BLOCK.  Powers of two up to 4096 take the tiled
  *        kernels (compress: with a 16-byte aligned payload; decompress:
  *        with a 16-byte aligned payload when bits < U; values need only
- *        their element alignment); other
- *        cases take slower generic kernels (compress: one thread per block
- *        up to B = 16, else a block-maximum pass and one thread per payload
- *        word; decompress and gather: one thread per value).
+ *        their element alignment).  Other cases take slower generic
+ *        kernels (compress: one thread per block up to B = 16, else a
+ *        block-maximum pass and one thread per payload word; decompress and
+ *        gather: one thread per value).
  *
  * Status codes follow the reference plugin: 0 ok, 1 unsupported dtype
  * (src/frsz2.cc:164, :181), 2 unsupported bits / block size with the
