@@ -314,8 +314,9 @@ launch_compress_generic(const void* in, void* out, const layout& L, std::uint32_
   }
   const std::uint64_t wpb = L.CB / static_cast<std::uint64_t>(U / 8);
   const std::uint64_t words = L.nb * wpb;
+  const bool narrow = words < (std::uint64_t{ 1 } << 32);
   hipLaunchKernelGGL((k_encode_words<T, U>), dim3(grid_for(words)), dim3(WG), 0, s, static_cast<const T*>(in),
-                     static_cast<std::uint8_t*>(out), L, static_cast<std::uint32_t>(wpb), words < (std::uint64_t{ 1 } << 32));
+                     static_cast<std::uint8_t*>(out), L, static_cast<std::uint32_t>(wpb), narrow);
   return hipGetLastError();
 }
 
