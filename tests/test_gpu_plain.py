@@ -151,7 +151,7 @@ def test_no_write_outside_stream(fz):
 
 
 @pytest.mark.parametrize("bits,B", [(16, 32), (12, 8), (21, 64), (12, 4096), (16, 5000), (32, 1)])
-def test_unaligned_pointers_take_generic_path(fz, port, bits, B):
+def test_values_off_16_byte_alignment(fz, port, bits, B):
     """Values 4 bytes off 16-byte alignment: compress loads them one by one
     in the 4096-value tile (power-of-two B up to 4096) or takes the generic
     kernels (B = 5000); decompress stores them through the strided decode."""
