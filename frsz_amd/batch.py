@@ -16,7 +16,7 @@ import ctypes
 import torch
 
 from . import _capi
-from ._capi import check
+from ._capi import Frsz2Error, check
 from .codec import compressed_size, dtype_code
 
 OP_COMPRESS = 0
@@ -70,22 +70,46 @@ _L.frsz2_batch_destroy.restype = ctypes.c_int
 _L.frsz2_batch_destroy.argtypes = [ctypes.c_void_p]
 
 
+def _require(t: torch.Tensor, what: str, dtypes=None) -> None:
+    # a job names raw device addresses: a tensor that is not on the GPU, not
+    # contiguous or of another dtype would be read or written out of bounds
+    if not t.is_cuda or not t.is_contiguous():
+        raise Frsz2Error(_capi.EARG, f"{what} must be a contiguous GPU tensor")
+    if dtypes is not None and t.dtype not in dtypes:
+        raise Frsz2Error(_capi.EARG, f"{what} must be of dtype {' or '.join(str(d) for d in dtypes)}")
+
+
+_VALUES = (torch.float32, torch.float64)
+
+
 def compress_job(x: torch.Tensor, out: torch.Tensor, bits: int, block: int):
+    _require(x, "values", _VALUES)
+    _require(out, "stream", (torch.uint8,))
     x = x.view(-1)
     return (Job(OP_COMPRESS, dtype_code(x.dtype), bits, 0, block, x.numel(), x.data_ptr(), out.data_ptr(), 0,
                 out.numel()), (x, out))
 
 
 def decompress_job(stream: torch.Tensor, out: torch.Tensor, bits: int, block: int):
+    _require(stream, "stream", (torch.uint8,))
+    _require(out, "output", _VALUES)
     out_flat = out.view(-1)
     return (Job(OP_DECOMPRESS, dtype_code(out.dtype), bits, 0, block, out_flat.numel(), stream.data_ptr(),
-                out_flat.data_ptr(), stream.numel(), 0), (stream, out))
+                out_flat.data_ptr(), stream.numel(), out_flat.numel() * out.element_size()), (stream, out))
 
 
 def gather_job(stream: torch.Tensor, n: int, out: torch.Tensor, idx: torch.Tensor, bits: int, block: int):
+    """Gather of the ``idx.numel()`` values ``idx`` names into ``out``, which
+    must hold at least that many values (the library checks the byte
+    capacity as well)."""
+    _require(stream, "stream", (torch.uint8,))
+    _require(out, "output", _VALUES)
+    _require(idx, "indices", (torch.int64, torch.uint64))
+    if out.numel() < idx.numel():
+        raise Frsz2Error(_capi.EARG, f"output holds {out.numel()} values, below the {idx.numel()} indices")
     idx = idx.view(-1)
     return (Job(OP_GATHER, dtype_code(out.dtype), bits, 0, block, n, stream.data_ptr(), out.data_ptr(),
-                stream.numel(), 0, idx.data_ptr(), idx.numel()), (stream, out, idx))
+                stream.numel(), out.numel() * out.element_size(), idx.data_ptr(), idx.numel()), (stream, out, idx))
 
 
 class BatchPlan:

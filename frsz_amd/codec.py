@@ -29,6 +29,8 @@ from ._capi import FLOAT32, FLOAT64, Frsz2Error, check
 __all__ = [
     "Frsz2Compressor",
     "Frsz2Error",
+    "PLUGIN_BLOCK_SIZES",
+    "plugin_check",
     "compressed_size",
     "compress",
     "compress_into",
@@ -118,10 +120,12 @@ def compress_blocks(values: torch.Tensor, stream: torch.Tensor, n: int, first_bl
     of that array's ``stream`` (compress_cpu_block, include/frsz2.hpp:1108-1179,
     over a run of blocks).  Runs over disjoint blocks together give the bytes
     of ``compress``.  Returns ``stream``."""
+    # the kernel writes into the caller's stream: a copy made to flatten a
+    # non-contiguous tensor would take the bytes and drop them
+    if stream.dtype != torch.uint8 or not stream.is_cuda or not stream.is_contiguous():
+        raise Frsz2Error(_capi.EARG, "stream must be a contiguous uint8 GPU tensor")
     v = _flat(values)
-    s = _flat(stream)
-    if s.dtype != torch.uint8:
-        raise Frsz2Error(_capi.EARG, "stream must be a uint8 GPU tensor")
+    s = stream.view(-1)
     check(_capi.lib.frsz2_compress_blocks(dtype_code(v.dtype), bits, block, n, first_block, v.data_ptr(), v.numel(),
                                           s.data_ptr(), s.numel(), _stream()))
     return stream
@@ -140,9 +144,36 @@ def decompress_blocks(stream: torch.Tensor, out: torch.Tensor, n: int, first_blo
     return out
 
 
+# The plugin's supported block sizes (src/frsz2.cc:86, :98, :112, :124); its
+# bit lists are 4..32 for float and 4..64 for double (:88-89, :100-103).
+PLUGIN_BLOCK_SIZES = (2, 4, 8, 16, 32, 64, 128, 256, 512, 1024)
+
+
+def _as_int(v: int) -> int:
+    # the plugin keeps both options as uint64 and hands them to the dispatch
+    # as int (src/frsz2.cc:91-92, :201-202): the low 32 bits, signed
+    v &= 0xFFFFFFFF
+    return v - (1 << 32) if v >= 1 << 31 else v
+
+
+def plugin_check(dtype: torch.dtype, bits: int, block: int) -> None:
+    """Raise the plugin's error for (dtype, bits, block): code 1 "unsupported
+    dtype" (src/frsz2.cc:164, :181), then code 2 with the dispatch's message,
+    bits checked before the block size (include/frsz2.hpp:1300, :1326)."""
+    code = dtype_code(dtype)
+    b, B = _as_int(bits), _as_int(block)
+    if not 4 <= b <= (32 if code == FLOAT32 else 64):
+        raise Frsz2Error(_capi.EPARAM, f"Unsupported number of bits: {b}")
+    if B not in PLUGIN_BLOCK_SIZES:
+        raise Frsz2Error(_capi.EPARAM, f"Unsupported exponent block size: {B}")
+
+
 class Frsz2Compressor:
     """The ``"frsz2"`` compressor with the plugin's option names and defaults
-    (src/frsz2.cc:28-64, :201-202)."""
+    (src/frsz2.cc:28-64, :201-202) and its supported lists: compress and
+    decompress reject what the plugin rejects, with its codes and messages
+    (``plugin_check``).  The module-level functions accept any bits 4..W and
+    any block size 1..2^24."""
 
     prefix = "frsz2"
     version = "0.0.2"
@@ -171,13 +202,16 @@ class Frsz2Compressor:
         return compressed_size(dtype, self.bits, self.max_exp_block_size, n)
 
     def compress(self, x: torch.Tensor) -> torch.Tensor:
-        return compress(x, self.bits, self.max_exp_block_size)
+        plugin_check(x.dtype, self.bits, self.max_exp_block_size)
+        return compress(x, _as_int(self.bits), _as_int(self.max_exp_block_size))
 
     def decompress(self, stream: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
-        return decompress_into(stream, out, self.bits, self.max_exp_block_size)
+        plugin_check(out.dtype, self.bits, self.max_exp_block_size)
+        return decompress_into(stream, out, _as_int(self.bits), _as_int(self.max_exp_block_size))
 
     def gather(self, stream: torch.Tensor, n: int, dtype: torch.dtype, idx: torch.Tensor) -> torch.Tensor:
-        return gather(stream, n, dtype, self.bits, self.max_exp_block_size, idx)
+        plugin_check(dtype, self.bits, self.max_exp_block_size)
+        return gather(stream, n, dtype, _as_int(self.bits), _as_int(self.max_exp_block_size), idx)
 
     def clone(self) -> "Frsz2Compressor":
         return Frsz2Compressor(self.bits, self.max_exp_block_size)

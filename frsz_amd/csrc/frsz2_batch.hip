@@ -543,6 +543,12 @@ validate_job(const frsz2_job& j, int idx)
     return set_error(FRSZ2_ECAPACITY, tag + "input stream below stream size");
   }
   const std::uintptr_t vs = j.dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  // decompress writes n values, gather m values: out_bytes must hold them
+  const std::uint64_t outv = j.op == FRSZ2_OP_DECOMPRESS ? j.n : j.op == FRSZ2_OP_GATHER ? j.m : 0;
+  if (j.op != FRSZ2_OP_COMPRESS && j.out_bytes / vs < outv) {
+    return set_error(FRSZ2_ECAPACITY, tag + "output capacity " + std::to_string(j.out_bytes) + " bytes below " +
+                                        std::to_string(outv) + " values");
+  }
   const void* vals = j.op == FRSZ2_OP_COMPRESS ? j.in : j.out;
   const void* strm = j.op == FRSZ2_OP_COMPRESS ? j.out : j.in;
   const std::uint64_t nvals = j.op == FRSZ2_OP_GATHER ? j.m : j.n;

@@ -343,6 +343,7 @@ admit(frsz2_server* S, slot& sl)
       j.in = S->pool + src->out_off;
       j.in_bytes = src->out_bytes;
       j.out = S->pool + off;
+      j.out_bytes = q.out_bytes;
       j.idx = q.r.idx;
       j.m = q.r.m;
     }

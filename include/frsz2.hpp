@@ -18,28 +18,40 @@
 //
 // compress_cpu_block / decompress_cpu_block keep their per-block meaning;
 // compress_gpu_blocks / decompress_gpu_blocks do the same for a run of
-// blocks in device memory.
+// blocks in device memory.  decompress_cpu_element decodes its one value in
+// place from the host stream (two or three loads, the same integer code the
+// kernels run); decompress_cpu_elements sends a batch of indices to the GPU.
 //
-// Compiled by hipcc, the view also offers, for use inside user kernels (the
-// view passed by value): decompress_gpu_element(idx)
-// (include/frsz2.hpp:1096-1104), and compress_gpu_block /
-// decompress_gpu_block, the per-block compress and decode by one thread.
+// The public static helpers of the reference view are kept:
+// compressed_to_uint_fp / uint_fp_to_compressed and write_shift_value /
+// retrieve_shift_value (include/frsz2.hpp:550-644).
+//
+// Compiled by hipcc, the header also offers the reference's device-side API:
+//   frsz::compress_gpu<C, block_size><<<grid, block_size>>>(d_values, view)
+//   frsz::decompress_gpu<C><<<grid, threads>>>(d_output, view)
+//                                        the free kernels of
+//                                        include/frsz2.hpp:1258-1288, correct
+//                                        for any 1-D grid (grid-stride loops)
+//   view.compress_gpu_function<block_size>(global_value_idx, value)
+//                                        the cooperative per-value compress
+//                                        of include/frsz2.hpp:916-1094
+//   view.decompress_gpu_element(idx)     include/frsz2.hpp:1096-1104
+// and, beyond the reference, compress_gpu_block / decompress_gpu_block, the
+// per-block compress and decode by one thread.
 #ifndef FRSZ2_MI355X_HPP
 #define FRSZ2_MI355X_HPP
 
 #include "frsz2_capi.h"
+#include "frsz2_device.hpp"
 
 #include <algorithm>
 #include <climits>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
-
-#if defined(__HIPCC__)
-#include "frsz2_device.hpp"
-#endif
 
 namespace frsz {
 
@@ -76,6 +88,100 @@ ceildiv(std::size_t a, std::size_t b)
   return a / b + (a % b != 0);
 }
 
+#if defined(__HIPCC__)
+// Launch shape of the cooperative compress of one workgroup of block_size
+// threads: it compresses `seg` exponent blocks per step (block_size / B, at
+// least one), through an LDS image of their payload and one key per block.
+template<class C, int block_size>
+struct wg_shape
+{
+  static constexpr std::size_t B = C::max_exp_block_size;
+  static constexpr std::size_t seg = block_size >= static_cast<int>(B) ? block_size / B : 1;
+  static constexpr std::size_t image_dwords = (seg * C::compressed_block_size_byte + 3) / 4;
+  // the LDS path is used while the image fits comfortably in a workgroup's LDS
+  static constexpr bool lds = image_dwords * 4 + seg * 4 <= 48 * 1024;
+};
+
+// One step of the cooperative compress: exponent blocks [k0, k0 + seg) of
+// the stream at `stream` (layout L), whose value i (array index) has the bit
+// pattern value(i).  Every thread of the 1-D workgroup calls it with the same
+// k0 (k0 < L.nb).  Block maxima are atomicMax-ed into keys (one per block),
+// codes are OR-ed into an LDS image of the blocks' payload at their bit
+// offsets, and the image is written out whole: every payload byte of the
+// blocks (codes past n are zero), as compress_cpu_block writes them
+// (include/frsz2.hpp:1108-1179).
+template<class C, int block_size, class Load>
+__device__ void
+wg_compress_step(std::uint8_t* __restrict__ stream,
+                 const frsz2::layout& L,
+                 std::uint64_t k0,
+                 Load value,
+                 std::uint32_t* keys,
+                 std::uint32_t* image)
+{
+  using T = typename C::fp_type;
+  using S = wg_shape<C, block_size>;
+  constexpr int U = C::uint_compressed_size_bit;
+  constexpr std::uint32_t B = static_cast<std::uint32_t>(S::B);
+  constexpr std::uint32_t CB = static_cast<std::uint32_t>(C::compressed_block_size_byte);
+  const std::uint32_t t = threadIdx.x;
+  const std::uint32_t nt = blockDim.x;
+  const std::uint64_t first = k0 * B;
+  const std::uint64_t left = L.n - first;
+  const std::uint32_t cnt = left < S::seg * B ? static_cast<std::uint32_t>(left) : static_cast<std::uint32_t>(S::seg * B);
+  const std::uint32_t nblk = (cnt + B - 1) / B;
+  const std::uint32_t nbytes = nblk * CB;
+  for (std::uint32_t i = t; i < nblk; i += nt) {
+    keys[i] = 0;
+  }
+  for (std::uint32_t i = t; i < (nbytes + 3) / 4; i += nt) {
+    image[i] = 0;
+  }
+  __syncthreads();
+  for (std::uint32_t i = t; i < cnt; i += nt) {
+    atomicMax(&keys[i / B], frsz2::max_key<T>(value(first + i)));
+  }
+  __syncthreads();
+  for (std::uint32_t b = t; b < nblk; b += nt) {
+    frsz2::store_block_exponent<U>(stream, k0 + b, frsz2::block_exponent_from_key<T, U>(keys[b]));
+  }
+  const std::uint32_t bits = static_cast<std::uint32_t>(C::bits_per_value);
+  for (std::uint32_t i = t; i < cnt; i += nt) {
+    const std::uint32_t b = i / B;
+    const int E = frsz2::block_exponent_from_key<T, U>(keys[b]);
+    const std::uint64_t code = static_cast<std::uint64_t>(frsz2::encode_value<T>(value(first + i), E, C::bits_per_value));
+    if (code != 0) {
+      // the code's bits [0, bits) go to image bits [pos, pos + bits)
+      const std::uint32_t pos = b * CB * 8u + (i - b * B) * bits;
+      const std::uint32_t q = pos >> 5;
+      const std::uint32_t o = pos & 31u;
+      const std::uint64_t lo = code << o;
+      atomicOr(&image[q], static_cast<std::uint32_t>(lo));
+      if (o + bits > 32) {
+        atomicOr(&image[q + 1], static_cast<std::uint32_t>(lo >> 32));
+      }
+      if (o + bits > 64) {
+        atomicOr(&image[q + 2], static_cast<std::uint32_t>(code >> (64 - o)));
+      }
+    }
+  }
+  __syncthreads();
+  std::uint8_t* dst = stream + L.R + k0 * CB;
+  if (CB % 4 == 0 && (reinterpret_cast<std::uintptr_t>(dst) & 3u) == 0) {
+    std::uint32_t* d4 = reinterpret_cast<std::uint32_t*>(dst);
+    for (std::uint32_t i = t; i < nbytes / 4; i += nt) {
+      d4[i] = image[i];
+    }
+  } else {
+    const std::uint8_t* src = reinterpret_cast<const std::uint8_t*>(image);
+    for (std::uint32_t i = t; i < nbytes; i += nt) {
+      dst[i] = src[i];
+    }
+  }
+  __syncthreads(); // keys and image are reused by the next step
+}
+#endif
+
 } // namespace detail
 
 template<int bits_per_value_, int max_exp_block_size_, class FpType_>
@@ -96,6 +202,7 @@ struct frsz2_compressor
   static_assert(bits_per_value <= 64, "maximum number for bits_per_value is 64.");
 
   static constexpr int uint_compressed_size_bit = sizeof(uint_compressed_type) * CHAR_BIT;
+  static constexpr int fp_bits = sizeof(fp_type) * CHAR_BIT;
   static constexpr int compressed_block_size_element_count =
     static_cast<int>(detail::ceildiv(static_cast<std::size_t>(max_exp_block_size) * bits_per_value,
                                      sizeof(uint_compressed_type) * CHAR_BIT));
@@ -104,7 +211,7 @@ struct frsz2_compressor
 
   // include/frsz2.hpp:534-540: exponent array rounded up to the next multiple
   // of 256 bytes (always 1..256 bytes of pad)
-  static constexpr std::size_t compute_exponent_block_size_byte(std::size_t number_elements)
+  static constexpr FRSZ2_HD std::size_t compute_exponent_block_size_byte(std::size_t number_elements)
   {
     constexpr std::size_t block_alignment{ 256 };
     const std::size_t exponent_element_size =
@@ -113,13 +220,76 @@ struct frsz2_compressor
   }
 
   // include/frsz2.hpp:542-548
-  static constexpr std::size_t compute_compressed_memory_size_byte(std::size_t number_elements)
+  static constexpr FRSZ2_HD std::size_t compute_compressed_memory_size_byte(std::size_t number_elements)
   {
     return compute_exponent_block_size_byte(number_elements) +
            detail::ceildiv(number_elements, max_exp_block_size) * compressed_block_size_byte;
   }
 
-  frsz2_compressor(std::uint8_t* data, std::size_t total_elements)
+  // ---- code <-> word helpers (public static members of the reference view,
+  // include/frsz2.hpp:550-644).  A code is the top bits_per_value bits of a
+  // W-bit word (W = bits of fp_type); in the payload it is stored as the low
+  // bits_per_value bits of a U-bit word (U = uint_compressed_size_bit), at a
+  // bit offset within that word, continuing into the next word when it does
+  // not fit.
+
+  static constexpr FRSZ2_HD uint_fp_type compressed_to_uint_fp(const uint_compressed_type& code)
+  {
+    return static_cast<uint_fp_type>(static_cast<uint_fp_type>(code) << (fp_bits - bits_per_value));
+  }
+
+  static constexpr FRSZ2_HD uint_compressed_type uint_fp_to_compressed(const uint_fp_type& word)
+  {
+    return static_cast<uint_compressed_type>(word >> (fp_bits - bits_per_value));
+  }
+
+  // Split the code of `input` at bit_offset of a word: first_val gets the
+  // bits that fall in the word, second_val those that spill into the next one
+  // (0 when none do).  With bits_per_value == U the code fills the word:
+  // first_val = the code, and bit_offset and second_val are left alone.
+  static constexpr FRSZ2_HD void write_shift_value(const int bit_offset,
+                                                   const uint_fp_type& input,
+                                                   uint_compressed_type& first_val,
+                                                   uint_compressed_type& second_val)
+  {
+    const uint_compressed_type code = uint_fp_to_compressed(input);
+    if (uint_compressed_size_bit == bits_per_value) {
+      first_val = code;
+      return;
+    }
+    first_val = static_cast<uint_compressed_type>(static_cast<std::uint64_t>(code) << bit_offset);
+    second_val = bit_offset + bits_per_value > uint_compressed_size_bit
+                   ? static_cast<uint_compressed_type>(static_cast<std::uint64_t>(code) >>
+                                                       (uint_compressed_size_bit - bit_offset))
+                   : uint_compressed_type{};
+  }
+
+  // The code at bit_offset of first_val (continued in second_val when it
+  // spills over), as a W-bit word; bits of the words above the code drop out.
+  static constexpr FRSZ2_HD uint_fp_type retrieve_shift_value(const int bit_offset,
+                                                              const uint_compressed_type& first_val,
+                                                              const uint_compressed_type& second_val)
+  {
+    if (uint_compressed_size_bit == bits_per_value) {
+      return compressed_to_uint_fp(first_val);
+    }
+    std::uint64_t r = static_cast<std::uint64_t>(first_val) >> bit_offset;
+    if (bit_offset + bits_per_value > uint_compressed_size_bit) {
+      r |= static_cast<std::uint64_t>(second_val) << (uint_compressed_size_bit - bit_offset);
+    }
+    return compressed_to_uint_fp(static_cast<uint_compressed_type>(r));
+  }
+
+  // The same from a pointer to the first word; the second word is read only
+  // when the code spills into it.
+  static constexpr FRSZ2_HD uint_fp_type retrieve_shift_value(const int bit_offset, const uint_compressed_type* values)
+  {
+    return uint_compressed_size_bit == bits_per_value || bit_offset + bits_per_value <= uint_compressed_size_bit
+             ? retrieve_shift_value(bit_offset, values[0], values[0])
+             : retrieve_shift_value(bit_offset, values[0], values[1]);
+  }
+
+  FRSZ2_HD frsz2_compressor(std::uint8_t* data, std::size_t total_elements)
     : data_{ data }
     , exponents_{ reinterpret_cast<exp_type*>(data) }
     , compressed_{ reinterpret_cast<uint_compressed_type*>(data + compute_exponent_block_size_byte(total_elements)) }
@@ -127,13 +297,13 @@ struct frsz2_compressor
   {
   }
 
-  uint_compressed_type* get_compressed_data() { return compressed_; }
-  const uint_compressed_type* get_compressed_data() const { return compressed_; }
-  const uint_compressed_type* get_const_compressed_data() const { return compressed_; }
-  exp_type* get_exponent_data() { return exponents_; }
-  const exp_type* get_exponent_data() const { return exponents_; }
-  const exp_type* get_const_exponent_data() const { return exponents_; }
-  std::size_t get_total_elements() const { return total_elements_; }
+  FRSZ2_HD uint_compressed_type* get_compressed_data() { return compressed_; }
+  FRSZ2_HD const uint_compressed_type* get_compressed_data() const { return compressed_; }
+  FRSZ2_HD const uint_compressed_type* get_const_compressed_data() const { return compressed_; }
+  FRSZ2_HD exp_type* get_exponent_data() { return exponents_; }
+  FRSZ2_HD const exp_type* get_exponent_data() const { return exponents_; }
+  FRSZ2_HD const exp_type* get_const_exponent_data() const { return exponents_; }
+  FRSZ2_HD std::size_t get_total_elements() const { return total_elements_; }
 
   // ---- host memory (the view's buffer and the values are host memory)
 
@@ -153,14 +323,17 @@ struct frsz2_compressor
     return 0;
   }
 
-  // decompress_cpu_element, include/frsz2.hpp:1199-1214
+  // decompress_cpu_element, include/frsz2.hpp:1199-1214: one value of the
+  // host stream, decoded in place (its block exponent and the one or two
+  // payload words holding its code) by the integer decode the kernels use.
+  // An index >= n yields 0 instead of reading past the stream.
   fp_type decompress_cpu_element(std::size_t idx) const
   {
-    fp_type out{};
-    const std::uint64_t i = idx;
-    detail::throw_if(frsz2_gather_host(detail::dtype_code<fp_type>(), bits_per_value, max_exp_block_size, data_,
-                                       stream_bytes(), total_elements_, &i, 1, &out));
-    return out;
+    if (idx >= total_elements_) {
+      return fp_type{};
+    }
+    const frsz2::layout L = frsz2::layout::make(bits_per_value, max_exp_block_size, total_elements_);
+    return frsz2::decode_element<fp_type, uint_compressed_size_bit>(data_, L, idx);
   }
 
   // compress_cpu_block, include/frsz2.hpp:1108-1179: block exp_block_id from
@@ -272,6 +445,34 @@ struct frsz2_compressor
       data_[i] = 0;
     }
   }
+
+  // compress_gpu_function<block_size>(global_value_idx, value),
+  // include/frsz2.hpp:916-1094: every thread of a 1-D workgroup of
+  // block_size threads calls it together, thread t with value
+  // global_value_idx = base + t of the array (base a multiple of block_size;
+  // values at or past n are ignored).  The workgroup then writes the
+  // exponent slots and payload blocks of its block_size / B exponent blocks.
+  // block_size must be a multiple of B.  It does not write the exponent pad:
+  // a kernel that compresses every block also calls clear_exponent_pad.
+  template<int block_size>
+  __device__ void compress_gpu_function(const std::size_t global_value_idx, const fp_type fp_input_value)
+  {
+    static_assert(block_size % max_exp_block_size == 0, "block_size must be a multiple of the exponent block size");
+    using shape = detail::wg_shape<frsz2_compressor, block_size>;
+    using uw = typename frsz2::fmt<fp_type>::uw;
+    __shared__ uw vals[block_size];
+    __shared__ std::uint32_t keys[shape::seg];
+    __shared__ std::uint32_t image[shape::image_dwords];
+    const std::size_t base = global_value_idx - threadIdx.x;
+    vals[threadIdx.x] = __builtin_bit_cast(uw, fp_input_value);
+    __syncthreads();
+    const frsz2::layout L = frsz2::layout::make(bits_per_value, max_exp_block_size, total_elements_);
+    if (base < total_elements_) {
+      detail::wg_compress_step<frsz2_compressor, block_size>(
+        data_, L, base / max_exp_block_size, [&](std::size_t i) { return vals[i - base]; }, keys, image);
+    }
+    __syncthreads(); // vals is reused by the workgroup's next call
+  }
 #endif
 
 private:
@@ -282,6 +483,58 @@ private:
   uint_compressed_type* const compressed_;
   const std::size_t total_elements_;
 };
+
+#if defined(__HIPCC__)
+// compress_gpu<C, block_size>, include/frsz2.hpp:1258-1274: compress the
+// view's n values from d_data into its (device) stream.  Correct for any 1-D
+// grid and any blockDim.x <= block_size: workgroups step through groups of
+// max(1, block_size / B) exponent blocks, grid-stride, each group compressed
+// cooperatively through LDS; workgroup 0 also clears the exponent pad.  The
+// stream bytes equal compress_cpu_impl's.  (Blocks whose payload image would
+// not fit the workgroup's LDS are compressed by one thread per block.)
+template<typename Frsz2Compressor, int block_size>
+__global__ __launch_bounds__(block_size) void compress_gpu(const typename Frsz2Compressor::fp_type* __restrict__ data,
+                                                           Frsz2Compressor compressor)
+{
+  using C = Frsz2Compressor;
+  using S = detail::wg_shape<C, block_size>;
+  using uw = typename frsz2::fmt<typename C::fp_type>::uw;
+  std::uint8_t* stream = reinterpret_cast<std::uint8_t*>(compressor.get_exponent_data());
+  const std::size_t n = compressor.get_total_elements();
+  if (blockIdx.x == 0) {
+    compressor.clear_exponent_pad(threadIdx.x, blockDim.x);
+  }
+  const frsz2::layout L = frsz2::layout::make(C::bits_per_value, C::max_exp_block_size, n);
+  if constexpr (S::lds) {
+    __shared__ std::uint32_t keys[S::seg];
+    __shared__ std::uint32_t image[S::image_dwords > 0 ? S::image_dwords : 1];
+    const std::uint64_t groups = (L.nb + S::seg - 1) / S::seg;
+    for (std::uint64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+      detail::wg_compress_step<C, block_size>(
+        stream, L, g * S::seg, [&](std::size_t i) { return __builtin_bit_cast(uw, data[i]); }, keys, image);
+    }
+  } else {
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < L.nb; k += stride) {
+      const std::size_t first = k * C::max_exp_block_size;
+      compressor.compress_gpu_block(k, data + first, n - first < C::max_exp_block_size ? n - first : C::max_exp_block_size);
+    }
+  }
+}
+
+// decompress_gpu<C>, include/frsz2.hpp:1276-1288: value i of the view's
+// stream into output[i], grid-stride, for any 1-D grid.
+template<typename Frsz2Compressor>
+__global__ void
+decompress_gpu(typename Frsz2Compressor::fp_type* const __restrict__ output, Frsz2Compressor compressor)
+{
+  const std::size_t n = compressor.get_total_elements();
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    output[i] = compressor.decompress_gpu_element(i);
+  }
+}
+#endif
 
 template<int... values>
 struct int_list_t

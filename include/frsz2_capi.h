@@ -235,7 +235,8 @@ typedef struct frsz2_job
   const void* in;       /* compress: values; decompress, gather: stream (device memory) */
   void* out;            /* compress: stream; decompress: n values; gather: m values */
   uint64_t in_bytes;    /* decompress, gather: bytes available at in */
-  uint64_t out_bytes;   /* compress: capacity at out */
+  uint64_t out_bytes;   /* bytes available at out: compress >= stream size,
+                           decompress >= n values, gather >= m values */
   const uint64_t* idx;  /* gather: m indices (device memory) */
   uint64_t m;           /* gather: index count */
 } frsz2_job;

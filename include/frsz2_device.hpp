@@ -14,9 +14,20 @@
 // floating-point operation takes part, so results are bit-exact.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
 #include <cstdint>
+
+// Every function here is plain integer code on bit patterns and pointers; it
+// compiles for the device under hipcc and for the host with any C++17
+// compiler (the drop-in header decodes single elements of host streams with
+// it).
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define FRSZ2_HD __host__ __device__
+#define FRSZ2_HDI __host__ __device__ __forceinline__
+#else
+#define FRSZ2_HD
+#define FRSZ2_HDI inline __attribute__((always_inline))
+#endif
 
 namespace frsz2 {
 
@@ -48,7 +59,7 @@ struct fmt<double>
 };
 
 // storage_t<bits> of include/frsz2.hpp:258-262
-__host__ __device__ constexpr int
+FRSZ2_HD constexpr int
 storage_bits(int bits)
 {
   return bits <= 8 ? 8 : bits <= 16 ? 16 : bits <= 32 ? 32 : 64;
@@ -59,7 +70,7 @@ storage_bits(int bits)
 // max(key) is the biased exponent of max|x| that include/frsz2.hpp:1121-1125
 // takes with std::max from 0 (which never lets a NaN in).
 template<typename T>
-__host__ __device__ __forceinline__ std::uint32_t
+FRSZ2_HDI std::uint32_t
 max_key(typename fmt<T>::uw u)
 {
   using F = fmt<T>;
@@ -79,7 +90,7 @@ max_key(typename fmt<T>::uw u)
 // double; it is then converted to the signed U-bit exp_type (:510), which
 // truncates for double with U = 8.
 template<typename T, int U>
-__host__ __device__ __forceinline__ int
+FRSZ2_HDI int
 block_exponent_from_key(std::uint32_t key)
 {
   int e = static_cast<int>(key) - fmt<T>::BIAS;
@@ -101,7 +112,7 @@ block_exponent_from_key(std::uint32_t key)
 // result that is zero anyway into zero).  Valid for E >= exponent(x), which
 // holds for every value of a block whose exponent was taken from its max.
 template<typename T>
-__host__ __device__ __forceinline__ typename fmt<T>::uw
+FRSZ2_HDI typename fmt<T>::uw
 encode_value(typename fmt<T>::uw u, int E, int bits)
 {
   using F = fmt<T>;
@@ -129,7 +140,7 @@ encode_value(typename fmt<T>::uw u, int E, int bits)
 // the block (E >= exponent(x)), so only the right shift and its clamp
 // remain.  Same code as encode_value for every such value, NaN included.
 template<typename T>
-__host__ __device__ __forceinline__ typename fmt<T>::uw
+FRSZ2_HDI typename fmt<T>::uw
 encode_value_short(typename fmt<T>::uw u, int E, int bits)
 {
   using F = fmt<T>;
@@ -159,12 +170,12 @@ encode_value_short(typename fmt<T>::uw u, int E, int bits)
 }
 
 // count of leading zeros of a W-bit value known to be non-zero
-__host__ __device__ __forceinline__ int
+FRSZ2_HDI int
 clz_nz(std::uint32_t v)
 {
   return __builtin_clz(v);
 }
-__host__ __device__ __forceinline__ int
+FRSZ2_HDI int
 clz_nz(std::uint64_t v)
 {
   return __builtin_clzll(v);
@@ -176,7 +187,7 @@ clz_nz(std::uint64_t v)
 // float and double (include/frsz2.hpp:180-192 on LP64), so a zero code in a
 // block with E + bias >= 31 decodes to +-2^(E-31); that is reproduced here.
 template<typename T>
-__host__ __device__ __forceinline__ typename fmt<T>::uw
+FRSZ2_HDI typename fmt<T>::uw
 decode_value(typename fmt<T>::uw code, int E, int bits)
 {
   using F = fmt<T>;
@@ -203,13 +214,13 @@ decode_value(typename fmt<T>::uw code, int E, int bits)
 // countl_zero(x) for x != 0 and 31 for x == 0: the reference's host
 // countl_zero(0) == 32 less one, for both widths.  For 32 bits this is
 // min(v_ffbh, 31) (v_ffbh gives all ones for zero).
-__host__ __device__ __forceinline__ std::uint32_t
+FRSZ2_HDI std::uint32_t
 clz_zero31(std::uint32_t x)
 {
   const std::uint32_t f = x == 0 ? 32u : static_cast<std::uint32_t>(__builtin_clz(x));
   return f < 31u ? f : 31u;
 }
-__host__ __device__ __forceinline__ std::uint32_t
+FRSZ2_HDI std::uint32_t
 clz_zero31(std::uint64_t x)
 {
   return x == 0 ? 31u : static_cast<std::uint32_t>(__builtin_clzll(x));
@@ -224,7 +235,7 @@ clz_zero31(std::uint64_t x)
 // (denormal).  Sign and exponent are joined before one shift into place.
 // The result is decode_value's bit pattern, in about half the operations.
 template<typename T>
-__host__ __device__ __forceinline__ typename fmt<T>::uw
+FRSZ2_HDI typename fmt<T>::uw
 decode_value_short(typename fmt<T>::uw code, int E, int bits)
 {
   using F = fmt<T>;
@@ -242,7 +253,7 @@ decode_value_short(typename fmt<T>::uw code, int E, int bits)
 
 // Sign-extended exponent of block k (exponents start at the stream base).
 template<int U>
-__device__ __forceinline__ int
+FRSZ2_HDI int
 load_block_exponent(const std::uint8_t* __restrict__ stream, std::uint64_t k)
 {
   if (U == 8) {
@@ -308,7 +319,7 @@ struct word<64>
 // (retrieve_shift_value, include/frsz2.hpp:615-627).  Never reads outside the
 // block.
 template<int U>
-__device__ __forceinline__ std::uint64_t
+FRSZ2_HDI std::uint64_t
 load_code(const std::uint8_t* __restrict__ blk, std::uint64_t j, int bits)
 {
   using wt = typename word<U>::type;
@@ -334,7 +345,7 @@ struct layout
   int bits;
   int U;
 
-  __host__ __device__ static layout make(int bits, std::uint64_t B, std::uint64_t n)
+  FRSZ2_HD static layout make(int bits, std::uint64_t B, std::uint64_t n)
   {
     layout L{};
     L.n = n;
@@ -348,13 +359,13 @@ struct layout
     L.CB = (bb / L.U + (bb % L.U != 0)) * static_cast<std::uint64_t>(L.U / 8);
     return L;
   }
-  __host__ __device__ std::uint64_t total_bytes() const { return R + nb * CB; }
+  FRSZ2_HD std::uint64_t total_bytes() const { return R + nb * CB; }
 };
 
 // Exponent slot k of a stream: the block exponent as a signed U-bit integer
 // (exp_type of include/frsz2.hpp:510, written at :1128).
 template<int U>
-__device__ __forceinline__ void
+FRSZ2_HDI void
 store_block_exponent(std::uint8_t* __restrict__ stream, std::uint64_t k, int e)
 {
   using wt = typename word<U>::type;
@@ -367,7 +378,7 @@ store_block_exponent(std::uint8_t* __restrict__ stream, std::uint64_t k, int e)
 // slot k and all CB payload bytes of block k; codes past `count` are zero.
 // The exponent pad is not touched.  T = float/double, U = storage_bits(bits).
 template<typename T, int U>
-__device__ inline void
+FRSZ2_HD inline void
 encode_block(std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t k, const T* __restrict__ values,
              std::uint64_t count)
 {
@@ -402,7 +413,7 @@ encode_block(std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t k
 // In-kernel decode of the first `count` values of exponent block k by one
 // thread (decompress_cpu_block, include/frsz2.hpp:1217-1240).
 template<typename T, int U>
-__device__ inline void
+FRSZ2_HD inline void
 decode_block(const std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t k, T* __restrict__ out,
              std::uint64_t count)
 {
@@ -418,7 +429,7 @@ decode_block(const std::uint8_t* __restrict__ stream, const layout& L, std::uint
 // include/frsz2.hpp:1199-1214), callable from any kernel.  T = float/double,
 // U = storage_bits(L.bits).
 template<typename T, int U>
-__device__ __forceinline__ T
+FRSZ2_HDI T
 decode_element(const std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t idx)
 {
   // a shift for power-of-two B (the branch is uniform when L is), else a

@@ -2,6 +2,9 @@
 
 * libdevmath_host.so: the device value arithmetic of include/frsz2_device.hpp
   compiled for the host, which tests/test_devmath.py checks against the oracle;
+* libheader_host.so: the drop-in header compiled by g++ alone, whose
+  host-side element decode and static helpers tests/test_header_host.py
+  checks on the golden vectors;
 * test_header_api: the C++ header-API program (include/frsz2.hpp) that
   tests/test_gpu_header_api.py runs, linked against frsz_amd/lib/libfrsz2_hip.so
   (build that first).
@@ -33,6 +36,10 @@ def _run(cmd):
 def build_test_helpers() -> None:
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
           f"-I{INCLUDE}", "-o", os.path.join(HERE, "libdevmath_host.so"), os.path.join(HERE, "devmath_host.cpp")])
+    # the drop-in header in plain host C++ (g++, no HIP headers)
+    _run(["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", "-o",
+          os.path.join(HERE, "libheader_host.so"), os.path.join(HERE, "header_host.cpp"), f"-L{LIBDIR}",
+          "-lfrsz2_hip", "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     _run([hipcc, "--offload-arch=gfx950", "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", f"-I{INCLUDE}", "-o", os.path.join(HERE, "test_header_api"),

@@ -72,6 +72,77 @@ block_decode_kernel(C view, typename C::fp_type* out, std::size_t n)
   }
 }
 
+// a user kernel built on the cooperative per-value compress: workgroups of
+// BS threads step through the array, each thread passing its value
+template<class C, int BS>
+__global__ __launch_bounds__(BS) void function_kernel(C view, const typename C::fp_type* in, std::size_t n)
+{
+  if (blockIdx.x == 0) {
+    view.clear_exponent_pad(threadIdx.x, blockDim.x);
+  }
+  const std::size_t end = (n + BS - 1) / BS * BS;
+  for (std::size_t base = static_cast<std::size_t>(blockIdx.x) * BS; base < end;
+       base += static_cast<std::size_t>(gridDim.x) * BS) {
+    const std::size_t i = base + threadIdx.x;
+    view.template compress_gpu_function<BS>(i, i < n ? in[i] : typename C::fp_type{});
+  }
+}
+
+// the reference header's free kernels, launched as the reference's GPU test
+// launches them (test/test_frsz_gpu.cu:325-341, 369-370) and with other
+// grids (grid-stride loops, fewer threads than block_size), into buffers of
+// other contents
+template<class C>
+void
+check_free_kernels(C dev_view, const typename C::fp_type* d_in, typename C::fp_type* d_out, std::uint8_t* d_stream,
+                   std::size_t n, const std::vector<std::uint8_t>& stream, const std::vector<typename C::fp_type>& out,
+                   const char* tname)
+{
+  using T = typename C::fp_type;
+  constexpr int B = C::max_exp_block_size;
+  constexpr int bits = C::bits_per_value;
+  const std::size_t size = stream.size();
+  std::vector<std::uint8_t> dstream(size);
+  std::vector<T> dout(n);
+  constexpr int comp_threads = 4 * B;
+  constexpr int blocks_per_tb = 512 / B > 1 ? 512 / B : 1;
+  constexpr int decomp_threads = blocks_per_tb * B;
+  struct grid
+  {
+    unsigned g, t;
+  };
+  const grid cgrids[] = { { static_cast<unsigned>((n + comp_threads - 1) / comp_threads), comp_threads },
+                          { 3, comp_threads },
+                          { 1, comp_threads / 2 > 0 ? comp_threads / 2 : 1 } };
+  for (const grid& g : cgrids) {
+    HIP_OK(hipMemset(d_stream, 0xFF, size));
+    frsz::compress_gpu<C, comp_threads><<<dim3(g.g), dim3(g.t)>>>(d_in, dev_view);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+    CHECK(dstream == stream, "%s b%d B%d: compress_gpu<C,%d><<<%u,%u>>> stream differs", tname, bits, B, comp_threads,
+          g.g, g.t);
+  }
+  const grid dgrids[] = { { static_cast<unsigned>((n + decomp_threads - 1) / decomp_threads),
+                            static_cast<unsigned>(decomp_threads) },
+                          { 2, 96 } };
+  for (const grid& g : dgrids) {
+    HIP_OK(hipMemset(d_out, 0x7F, n * sizeof(T)));
+    frsz::decompress_gpu<C><<<dim3(g.g), dim3(g.t)>>>(d_out, dev_view);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+    CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "%s b%d B%d: decompress_gpu<<<%u,%u>>> differs",
+          tname, bits, B, g.g, g.t);
+  }
+  // compress_gpu_function in a user kernel, workgroups of 4B threads
+  for (unsigned g : { static_cast<unsigned>((n + comp_threads - 1) / comp_threads), 2u }) {
+    HIP_OK(hipMemset(d_stream, 0xC3, size));
+    function_kernel<C, comp_threads><<<dim3(g), dim3(comp_threads)>>>(dev_view, d_in, n);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+    CHECK(dstream == stream, "%s b%d B%d: compress_gpu_function (grid %u) stream differs", tname, bits, B, g);
+  }
+}
+
 template<int bits, int B, class T>
 void
 run_case(const std::vector<T>& in, const std::string& outdir)
@@ -138,6 +209,7 @@ run_case(const std::vector<T>& in, const std::string& outdir)
   HIP_OK(hipDeviceSynchronize());
   HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
   CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "decompress_gpu_element differs");
+  check_free_kernels<C>(dev_view, d_in, d_out, d_stream, n, stream, out, tname);
   // in-kernel per-block compress and decode
   {
     const std::size_t nb = (n + B - 1) / B;
@@ -167,6 +239,95 @@ run_case(const std::vector<T>& in, const std::string& outdir)
   HIP_OK(hipFree(d_in));
   HIP_OK(hipFree(d_out));
   HIP_OK(hipFree(d_stream));
+}
+
+// beyond the reference's list: bits = W with B = 1 (lossless,
+// test/test_frsz_gpu.cu:414-441), a block larger than the workgroup, and
+// non-power-of-two blocks, on the free kernels against the host-API stream
+template<int bits, int B, class T>
+void
+free_kernels_only(const std::vector<T>& in)
+{
+  using C = frsz::frsz2_compressor<bits, B, T>;
+  const std::size_t n = in.size();
+  const std::size_t size = C::compute_compressed_memory_size_byte(n);
+  const char* tname = sizeof(T) == 4 ? "float" : "double";
+  std::vector<std::uint8_t> stream(size, 0);
+  C host_view(stream.data(), n);
+  host_view.compress_cpu_impl(in.data());
+  std::vector<T> out(n);
+  host_view.decompress_cpu_impl(out.data());
+  T* d_in = nullptr;
+  T* d_out = nullptr;
+  std::uint8_t* d_stream = nullptr;
+  HIP_OK(hipMalloc(&d_in, n * sizeof(T)));
+  HIP_OK(hipMalloc(&d_out, n * sizeof(T)));
+  HIP_OK(hipMalloc(&d_stream, size));
+  HIP_OK(hipMemcpy(d_in, in.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  C dev_view(d_stream, n);
+  if constexpr (4 * B <= 1024) {
+    check_free_kernels<C>(dev_view, d_in, d_out, d_stream, n, stream, out, tname);
+  } else {
+    // a block of more values than the workgroup has threads
+    std::vector<std::uint8_t> dstream(size);
+    HIP_OK(hipMemset(d_stream, 0xFF, size));
+    frsz::compress_gpu<C, 256><<<dim3(5), dim3(256)>>>(d_in, dev_view);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+    CHECK(dstream == stream, "%s b%d B%d: compress_gpu<C,256> stream differs", tname, bits, B);
+    std::vector<T> dout(n);
+    frsz::decompress_gpu<C><<<dim3(7), dim3(256)>>>(d_out, dev_view);
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+    CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "%s b%d B%d: decompress_gpu differs", tname,
+          bits, B);
+  }
+  if (bits == static_cast<int>(sizeof(T) * 8) && B == 1) {
+    CHECK(std::memcmp(out.data(), in.data(), n * sizeof(T)) == 0, "%s: bits = W, B = 1 not lossless", tname);
+  }
+  HIP_OK(hipFree(d_in));
+  HIP_OK(hipFree(d_out));
+  HIP_OK(hipFree(d_stream));
+}
+
+void
+check_wide_cases()
+{
+  // random finite values of both signs over a wide exponent range, a few
+  // zeros and subnormals
+  std::vector<float> f(100003);
+  std::vector<double> d(50021);
+  std::uint64_t st = 0x9E3779B97F4A7C15ull;
+  auto next = [&st]() {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return st;
+  };
+  for (auto& v : f) {
+    const std::uint32_t r = static_cast<std::uint32_t>(next());
+    std::uint32_t u = (r & 0x807fffffu) | ((60u + (r >> 8) % 136u) << 23);
+    if (r % 97 == 0) {
+      u &= 0x807fffffu; // subnormal or zero
+    }
+    std::memcpy(&v, &u, 4);
+  }
+  for (auto& v : d) {
+    const std::uint64_t r = next();
+    std::uint64_t u = (r & 0x800fffffffffffffull) | ((900ull + (r >> 20) % 250ull) << 52);
+    if (r % 89 == 0) {
+      u &= 0x800fffffffffffffull;
+    }
+    std::memcpy(&v, &u, 8);
+  }
+  free_kernels_only<32, 1, float>(f);
+  free_kernels_only<64, 1, double>(d);
+  free_kernels_only<12, 3, float>(f);
+  free_kernels_only<21, 7, float>(f);
+  free_kernels_only<40, 100, double>(d);
+  free_kernels_only<16, 256, float>(f);
+  free_kernels_only<24, 1024, float>(f);
+  free_kernels_only<53, 2048, double>(d);
 }
 
 void
@@ -223,6 +384,7 @@ main(int argc, char** argv)
   run_case<15, 8, float>(vf, outdir);
   run_case<9, 4, float>(vf, outdir);
   run_case<9, 5, float>(vf, outdir);
+  check_wide_cases();
   check_dispatch();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "OK", g_fail);
   return g_fail ? 1 : 0;

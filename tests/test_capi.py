@@ -53,14 +53,67 @@ def test_check_rejects(dtype, bits, block, rc, msg):
     assert capi.last_error() == msg
 
 
+DISPATCH_ERRORS = json.load(open(os.path.join(GOLDEN, "dispatch_errors.json")))
+# the bits list the reference dispatch ran with when the fixture was written
+# (tests/golden/make_golden.py); its block list is the plugin's
+FIXTURE_BITS = (4, 8, 12, 16, 21, 32)
+
+
 def test_messages_match_reference_dispatch():
-    # the reference's runtime dispatch messages (include/frsz2.hpp:1300, :1326)
-    for e in json.load(open(os.path.join(GOLDEN, "dispatch_errors.json"))):
+    # the C ABI accepts a superset (any bits 4..W, any block 1..2^24); where it
+    # rejects, code and message are the reference dispatch's
+    # (include/frsz2.hpp:1300, :1326)
+    for e in DISPATCH_ERRORS:
         bits, block = e["bits"], e["block"]
         if bits == 5 or block in (3, 2048):
-            continue  # supported here: any bits 4..W and any block size 1..2^24
+            continue
         assert capi.lib.frsz2_check(0, bits, block) == e["rc"]
         assert capi.last_error() == e["message"]
+
+
+@pytest.mark.parametrize("case", DISPATCH_ERRORS, ids=lambda e: f"b{e['bits']}-B{e['block']}")
+def test_plugin_check_matches_reference_dispatch(case):
+    """The Python plugin surface rejects exactly what the reference plugin
+    does (src/frsz2.cc:83-134), with the dispatch's code and message.  The
+    fixture's bits 5 is outside the bits list it was written with but inside
+    the plugin's 4..32, so the plugin accepts it."""
+    import torch
+
+    from frsz_amd.codec import Frsz2Error, plugin_check
+
+    bits, block = case["bits"], case["block"]
+    if bits not in FIXTURE_BITS and 4 <= bits <= 32 and block in (2, 4, 8, 16, 32, 64, 128, 256, 512, 1024):
+        plugin_check(torch.float32, bits, block)
+        return
+    with pytest.raises(Frsz2Error) as ei:
+        plugin_check(torch.float32, bits, block)
+    assert (ei.value.code, ei.value.message) == (case["rc"], case["message"])
+
+
+def test_plugin_check_lists():
+    import torch
+
+    from frsz_amd.codec import Frsz2Error, PLUGIN_BLOCK_SIZES, plugin_check
+
+    for B in range(0, 2050):
+        ok = B in PLUGIN_BLOCK_SIZES
+        try:
+            plugin_check(torch.float64, 16, B)
+            assert ok, B
+        except Frsz2Error as e:
+            assert not ok and e.code == 2 and e.message == f"Unsupported exponent block size: {B}"
+    for dt, W in ((torch.float32, 32), (torch.float64, 64)):
+        for b in range(-1, 70):
+            try:
+                plugin_check(dt, b, 32)
+                assert 4 <= b <= W, b
+            except Frsz2Error as e:
+                assert not 4 <= b <= W and e.code == 2 and e.message == f"Unsupported number of bits: {b}"
+    with pytest.raises(Frsz2Error) as ei:
+        plugin_check(torch.float16, 16, 32)
+    assert (ei.value.code, ei.value.message) == (1, "unsupported dtype")
+    # bits go through int like the plugin's dispatch call: the low 32 bits
+    plugin_check(torch.float32, (1 << 32) + 16, 32)
 
 
 @pytest.mark.parametrize("dtype,bits,B,n", [(0, 16, 32, 1 << 20), (1, 32, 32, 512 ** 3), (0, 21, 16, 100_000_037),
@@ -83,6 +136,45 @@ def test_capacity_and_pointer_errors():
     assert capi.lib.frsz2_compress(0, 16, 32, bad, 1000, fake, 1 << 20, None) == capi.EARG
     assert "misaligned" in capi.last_error()
     assert capi.lib.frsz2_compress(0, 16, 32, fake, 1000, None, 1 << 20, None) == capi.EARG
+
+
+def test_batch_rejects_undersized_outputs():
+    """A batch job's out_bytes must hold its output: the stream for compress,
+    n values for decompress, m values for gather (validated before any
+    device call)."""
+    from frsz_amd.batch import BatchConfig, Job
+
+    fake = 1 << 20
+    h = ctypes.c_void_p()
+
+    def create(job):
+        arr = (Job * 1)(job)
+        return capi.lib.frsz2_batch_create(arr, 1, None, ctypes.byref(h))
+
+    from oracle.oracle import compressed_bytes
+
+    S = compressed_bytes(16, 32, 1000)
+    assert create(Job(0, 0, 16, 0, 32, 1000, fake, fake, 0, S - 1)) == capi.ECAPACITY
+    assert create(Job(1, 0, 16, 0, 32, 1000, fake, fake, S, 3999)) == capi.ECAPACITY
+    assert "below 1000 values" in capi.last_error()
+    assert create(Job(1, 1, 16, 0, 32, 1000, fake, fake, S, 7999)) == capi.ECAPACITY
+    assert create(Job(2, 0, 16, 0, 32, 1000, fake, fake, S, 4 * 99, fake, 100)) == capi.ECAPACITY
+    assert "below 100 values" in capi.last_error()
+    assert create(Job(2, 0, 16, 0, 32, 1000, fake, fake, S, 400, fake + 4, 100)) == capi.EARG
+    assert BatchConfig  # the structure the plan functions take
+
+
+def test_batch_job_helpers_check_tensors():
+    import torch
+
+    from frsz_amd import batch
+    from frsz_amd.codec import Frsz2Error
+
+    cpu = torch.zeros(8)
+    with pytest.raises(Frsz2Error):
+        batch.gather_job(torch.zeros(64, dtype=torch.uint8), 8, cpu, torch.zeros(4, dtype=torch.int64), 16, 32)
+    with pytest.raises(Frsz2Error):
+        batch.compress_job(cpu, torch.zeros(64, dtype=torch.uint8), 16, 32)
 
 
 def test_python_plugin_surface_defaults():

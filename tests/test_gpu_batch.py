@@ -170,7 +170,9 @@ def test_autotune_keeps_bytes(fz):
     outs = [torch.empty_like(s) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
     chosen = plan.autotune(reps=1)
-    assert chosen["tile_shape"] in (0, 1, 2) and chosen["workgroups"] > 0
+    # whichever candidate wins the timing, the plan keeps a valid shape and the bytes
+    assert chosen["tile_shape"] in (0, 1, 2, 3)
+    assert chosen == plan.config
     for o in outs:
         o.zero_()
     plan.launch()
