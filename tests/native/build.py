@@ -5,6 +5,10 @@
 * libheader_host.so: the drop-in header compiled by g++ alone, whose
   host-side element decode and static helpers tests/test_header_host.py
   checks on the golden vectors;
+* libfrsz2_plugin_double.so: the "frsz2" LibPressio plugin
+  (frsz_amd/plugin/frsz2_plugin.cc) compiled against the LibPressio test
+  double in tests/pressio_double, plus tests/native/plugin_driver.cc, for
+  tests/test_plugin.py and tests/test_gpu_plugin.py;
 * test_header_api: the C++ header-API program (include/frsz2.hpp) that
   tests/test_gpu_header_api.py runs, linked against frsz_amd/lib/libfrsz2_hip.so
   (build that first).
@@ -40,6 +44,12 @@ def build_test_helpers() -> None:
     _run(["g++", "-std=c++17", "-O1", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", "-o",
           os.path.join(HERE, "libheader_host.so"), os.path.join(HERE, "header_host.cpp"), f"-L{LIBDIR}",
           "-lfrsz2_hip", "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
+    # the "frsz2" LibPressio plugin source against the LibPressio test double,
+    # with the C entry points the plugin tests drive it through
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", f"-I{INCLUDE}",
+          f"-I{os.path.join(ROOT, 'tests', 'pressio_double')}", "-o", os.path.join(HERE, "libfrsz2_plugin_double.so"),
+          os.path.join(ROOT, "frsz_amd", "plugin", "frsz2_plugin.cc"), os.path.join(HERE, "plugin_driver.cc"),
+          f"-L{LIBDIR}", "-lfrsz2_hip", "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     _run([hipcc, "--offload-arch=gfx950", "-std=c++17", "-O2", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", f"-I{INCLUDE}", "-o", os.path.join(HERE, "test_header_api"),
