@@ -69,14 +69,9 @@ def bench_batched(count, steps, seed=4, target_input_bytes=None):
 
     import frsz_amd
     from frsz_amd import batch
-    from frsz_amd.workloads import batched_lengths, smooth_signal_torch
+    from frsz_amd.workloads import batched_plan_arrays, smooth_signal_torch
 
-    lengths, is_f64, bits = batched_lengths(count, seed=seed)
-    scale = 1.0
-    if target_input_bytes:
-        full = float(np.sum(lengths * np.where(is_f64, 8, 4)))
-        scale = full / target_input_bytes
-        lengths = np.maximum(1, np.round(lengths / scale)).astype(np.int64)
+    lengths, is_f64, bits, scale = batched_plan_arrays(count, seed=seed, target_input_bytes=target_input_bytes)
     B = 32
     xs, streams, outs = [], [], []
     alg = 0
@@ -388,80 +383,33 @@ def bench_host(x_host, bits, block, want_stream, want_y, reps=3):
             "bit_exact_vs_device_path": same}
 
 
-def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_options=None):
+def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_options=None, source_gib=4):
     """Poisson arrivals at `rate` requests/s for `seconds`: 50% compress, 30%
     full decompress, 20% random-access reads of 1-4096 values from resident
     streams; sizes log-uniform 4 KiB-256 MiB, float32 / float64 (half each),
-    bits from {16, 32}, B = 32; HBM budget `budget_gib` GiB.  Open-loop replay
-    through the serving runtime; latency = completion - scheduled arrival.
-    Every `verify_every`-th request keeps its output, which is then compared
-    with the same request run alone through the single-array API."""
+    bits from {16, 32}, B = 32; HBM budget `budget_gib` GiB.  Compress
+    requests read slices of two `source_gib` GiB source buffers (one per
+    dtype), far larger than the on-die caches.  Open-loop replay through the
+    serving runtime; latency = completion - scheduled arrival.  Every
+    `verify_every`-th request keeps its output, which is then compared with
+    the same request run alone through the single-array API."""
     import numpy as np
     import torch
 
     import frsz_amd
     from frsz_amd import serve
-    from frsz_amd.workloads import smooth_signal_torch
+    from frsz_amd.workloads import serving_requests, serving_trace, smooth_signal_torch
 
-    rng = np.random.default_rng(seed)
-    nreq = int(seconds * rate)
-    arrival = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    kind = rng.choice(3, size=nreq, p=[0.5, 0.3, 0.2])
-    kind[0] = 0
-    f64 = rng.random(nreq) < 0.5
-    bits = np.where(rng.random(nreq) < 0.5, 16, 32)
-    nbytes = np.exp(rng.uniform(np.log(4096), np.log(256 << 20), nreq))
+    src_values = {False: (source_gib << 30) // 4, True: (source_gib << 30) // 8}
+    tr = serving_trace(seconds, rate, src_values, seed=seed)
+    kind, f64, bits, n, offs, source, m, arrival = (tr[k] for k in ("kind", "f64", "bits", "n", "offset", "source",
+                                                                  "m", "arrival"))
+    nreq = kind.size
     B = 32
-    src = {False: smooth_signal_torch((256 << 20) // 4, seed=77, dtype=torch.float32),
-           True: smooth_signal_torch((256 << 20) // 8, seed=78, dtype=torch.float64)}
-    trace = np.zeros(nreq, dtype=serve.REQUEST_DTYPE)
-    n = np.zeros(nreq, dtype=np.int64)
-    offs = np.zeros(nreq, dtype=np.int64)
-    source = np.zeros(nreq, dtype=np.int64)
-    recent = []  # indices of the latest compress requests
-    for i in range(nreq):
-        if kind[i] != 0 and not recent:
-            kind[i] = 0
-        if kind[i] == 0:
-            vs = 8 if f64[i] else 4
-            n[i] = max(1, int(nbytes[i]) // vs)
-            total = src[bool(f64[i])].numel()
-            offs[i] = int(rng.integers(0, (total - n[i]) // 4 + 1)) * 4
-            recent.append(i)
-            if len(recent) > 64:
-                recent.pop(0)
-        else:
-            s = recent[int(rng.integers(0, len(recent)))]
-            source[i] = s
-            n[i] = n[s]
-            f64[i] = f64[s]
-            bits[i] = bits[s]
-    m = np.where(kind == 2, rng.integers(1, 4097, nreq), 0)
-    # gather indices, generated on the GPU in one pass
-    gi = np.nonzero(kind == 2)[0]
-    idx_all = None
-    idx_off = np.zeros(nreq, dtype=np.int64)
-    if gi.size:
-        counts = torch.from_numpy(m[gi]).cuda()
-        lim = torch.from_numpy(n[gi]).cuda()
-        rep = torch.repeat_interleave(lim, counts)
-        idx_all = (torch.rand(rep.numel(), dtype=torch.float64, device="cuda") * rep).long()
-        idx_all = torch.minimum(idx_all, rep - 1)
-        del rep
-        idx_off[gi] = np.concatenate([[0], np.cumsum(m[gi])[:-1]])
+    src = {False: smooth_signal_torch(src_values[False], seed=77, dtype=torch.float32),
+           True: smooth_signal_torch(src_values[True], seed=78, dtype=torch.float64)}
     keep = (np.arange(nreq) % verify_every) == 0
-    trace["op"] = kind
-    trace["dtype"] = f64.astype(np.int32)
-    trace["bits"] = bits
-    trace["flags"] = np.where(keep, 0, serve.REQ_DISCARD)
-    trace["block"] = B
-    trace["n"] = n
-    base32, base64 = src[False].data_ptr(), src[True].data_ptr()
-    trace["values"] = np.where(kind == 0, np.where(f64, base64 + offs * 8, base32 + offs * 4), 0)
-    trace["source"] = source
-    trace["idx"] = np.where(kind == 2, (idx_all.data_ptr() if idx_all is not None else 0) + idx_off * 8, 0)
-    trace["m"] = m
-    trace["arrival"] = arrival
+    trace, idx_all, idx_off = serving_requests(tr, src, B, keep)
     torch.cuda.synchronize()
     print(f"bench: serving {nreq} requests over {seconds:.0f} s", file=sys.stderr, flush=True)
     opts = dict(slots=4, tile_shape=3)

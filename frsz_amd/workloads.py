@@ -83,3 +83,108 @@ def batched_lengths(count: int = 4096, lo: float = 1e3, hi: float = 1.6e7, seed:
     is_f64 = rng.random(count) < 0.4
     bits = np.where(rng.random(count) < 0.5, 16, 32)
     return lengths, is_f64, bits
+
+
+def serving_trace(seconds: float, rate: float, source_values: dict, seed: int = 5, lo_bytes: int = 4096,
+                  hi_bytes: int = 256 << 20, recent: int = 64):
+    """Request stream of the serving workload: Poisson arrivals at ``rate``
+    per second for ``seconds``; 50% compress, 30% full decompress, 20%
+    random-access reads of 1-4096 values; sizes log-uniform over
+    [lo_bytes, hi_bytes]; float32 / float64 half each; bits from {16, 32}.
+    A compress reads ``n`` values at offset ``offset`` (a multiple of 4
+    values) of a source buffer of ``source_values[is_f64]`` values; a
+    decompress or gather reads the stream of one of the ``recent`` latest
+    compress requests (``source``, a trace index) with its dtype, bits and n.
+
+    Returns numpy arrays: kind (0/1/2), f64, bits, n, offset, source, m,
+    arrival (seconds)."""
+    rng = np.random.default_rng(seed)
+    nreq = int(seconds * rate)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    kind = rng.choice(3, size=nreq, p=[0.5, 0.3, 0.2])
+    if nreq:
+        kind[0] = 0
+    f64 = rng.random(nreq) < 0.5
+    bits = np.where(rng.random(nreq) < 0.5, 16, 32)
+    nbytes = np.exp(rng.uniform(np.log(lo_bytes), np.log(hi_bytes), nreq))
+    n = np.zeros(nreq, dtype=np.int64)
+    offset = np.zeros(nreq, dtype=np.int64)
+    source = np.zeros(nreq, dtype=np.int64)
+    latest = []
+    for i in range(nreq):
+        if kind[i] != 0 and not latest:
+            kind[i] = 0
+        if kind[i] == 0:
+            vs = 8 if f64[i] else 4
+            n[i] = max(1, int(nbytes[i]) // vs)
+            total = int(source_values[bool(f64[i])])
+            offset[i] = int(rng.integers(0, (total - n[i]) // 4 + 1)) * 4
+            latest.append(i)
+            if len(latest) > recent:
+                latest.pop(0)
+        else:
+            s = latest[int(rng.integers(0, len(latest)))]
+            source[i] = s
+            n[i] = n[s]
+            f64[i] = f64[s]
+            bits[i] = bits[s]
+    m = np.where(kind == 2, rng.integers(1, 4097, nreq), 0)
+    return {"kind": kind, "f64": f64, "bits": bits, "n": n, "offset": offset, "source": source, "m": m,
+            "arrival": arrival}
+
+
+def serving_requests(tr: dict, sources: dict, block: int, keep, seed: int = 6):
+    """Request records (frsz_amd.serve.REQUEST_DTYPE) for a ``serving_trace``
+    over the device source buffers ``sources[is_f64]``.  Gather indices are
+    drawn on the GPU, uniform over each request's n.  ``keep`` (bool per
+    request) marks the requests whose output stays held; the rest discard it.
+    Returns (records, idx_all, idx_off): request i's indices are
+    idx_all[idx_off[i] : idx_off[i] + m[i]]."""
+    import torch
+
+    from .serve import REQ_DISCARD, REQUEST_DTYPE
+
+    kind, f64, n, m = tr["kind"], tr["f64"], tr["n"], tr["m"]
+    nreq = kind.size
+    gi = np.nonzero(kind == 2)[0]
+    idx_all = None
+    idx_off = np.zeros(nreq, dtype=np.int64)
+    if gi.size:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        counts = torch.from_numpy(m[gi]).cuda()
+        lim = torch.from_numpy(n[gi]).cuda()
+        rep = torch.repeat_interleave(lim, counts)
+        idx_all = (torch.rand(rep.numel(), dtype=torch.float64, device="cuda", generator=g) * rep).long()
+        idx_all = torch.minimum(idx_all, rep - 1)
+        del rep
+        idx_off[gi] = np.concatenate([[0], np.cumsum(m[gi])[:-1]])
+    rec = np.zeros(nreq, dtype=REQUEST_DTYPE)
+    rec["op"] = kind
+    rec["dtype"] = f64.astype(np.int32)
+    rec["bits"] = tr["bits"]
+    rec["flags"] = np.where(keep, 0, REQ_DISCARD)
+    rec["block"] = block
+    rec["n"] = n
+    base32, base64 = sources[False].data_ptr(), sources[True].data_ptr()
+    offs = tr["offset"]
+    rec["values"] = np.where(kind == 0, np.where(f64, base64 + offs * 8, base32 + offs * 4), 0)
+    rec["source"] = tr["source"]
+    rec["idx"] = np.where(kind == 2, (idx_all.data_ptr() if idx_all is not None else 0) + idx_off * 8, 0)
+    rec["m"] = m
+    rec["arrival"] = tr["arrival"]
+    return rec, idx_all, idx_off
+
+
+def batched_plan_arrays(count: int = 4096, seed: int = 4, target_input_bytes=None):
+    """The batched workload's arrays: ``batched_lengths``, optionally with
+    every length divided by one factor so that the input totals
+    ``target_input_bytes``.  Returns (lengths, is_f64, bits, scale); array i
+    holds ``smooth_signal_torch(lengths[i], seed=10_000 + i)`` in its dtype."""
+    lengths, is_f64, bits = batched_lengths(count, seed=seed)
+    scale = 1.0
+    if target_input_bytes:
+        full = float(np.sum(lengths * np.where(is_f64, 8, 4)))
+        scale = full / target_input_bytes
+        lengths = np.maximum(1, np.round(lengths / scale)).astype(np.int64)
+    return lengths, is_f64, bits, scale

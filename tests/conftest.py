@@ -53,3 +53,30 @@ def reference():
     if not reference_available():
         pytest.skip("oracle/_ref not built (needs the reference checkout)")
     return Reference()
+
+
+def oracle_codec(port, x, bits, B):
+    """The oracle's stream and decode of the device tensor ``x`` (computed on
+    the host from a copy of ``x``), returned as device tensors for comparison
+    with what the HIP path wrote."""
+    import torch
+
+    xh = x.detach().reshape(-1).cpu().numpy()
+    s = port.compress(xh, bits, B)
+    y = port.decompress(s, xh.size, xh.dtype, bits, B)
+
+    def dev(a, dtype):
+        # a fresh contiguous tensor, also for an empty result
+        t = torch.empty(a.size, dtype=dtype, device=x.device)
+        if a.size:
+            t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
+        return t
+
+    return dev(s, torch.uint8), dev(y, x.dtype)
+
+
+@pytest.fixture(scope="session")
+def port_mt(port):
+    """The oracle on every host core (block-parallel, byte-identical to one
+    thread), for the large cases."""
+    return port.with_threads(0)

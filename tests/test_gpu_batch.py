@@ -1,6 +1,6 @@
-"""Batched path: every job of a persistent, graph-captured batch must produce
-exactly the bytes of the plain single-array call (which the plain tests pin
-to the oracle), for any launch configuration and on every graph replay."""
+"""Batched path: every job of a graph-captured batch must produce exactly the
+oracle's bytes (stream, decode, gather) for its array, for any launch
+configuration and on every graph replay."""
 import numpy as np
 import pytest
 
@@ -40,12 +40,15 @@ def make_arrays(seed, count=60):
     return specs
 
 
-def plain_outputs(fz_mod, specs):
+def oracle_outputs(port, specs):
+    """The oracle's stream and decode of every array, as device tensors."""
+    from conftest import oracle_codec
+
     streams, values = [], []
     for x, bits, B in specs:
-        s = fz_mod.compress(x, bits, B)
+        s, y = oracle_codec(port, x, bits, B)
         streams.append(s)
-        values.append(fz_mod.decompress(s, x.numel(), x.dtype, bits, B))
+        values.append(y)
     return streams, values
 
 
@@ -60,10 +63,10 @@ CONFIGS = [None,
 
 @pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "static", "nograph", "chunk16", "plain-graph", "grouped",
                                              "balanced-odd"])
-def test_batch_matches_plain(fz, cfg):
+def test_batch_matches_oracle(fz, port, cfg):
     fz_mod, batch = fz
     specs = make_arrays(1)
-    ref_s, ref_v = plain_outputs(fz_mod, specs)
+    ref_s, ref_v = oracle_outputs(port, specs)
     outs = [torch.full_like(s, 0x5A) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
     plan.launch()
@@ -84,13 +87,13 @@ def test_batch_matches_plain(fz, cfg):
                                        (4, {"workgroups": 16, "tiles_per_claim": 0, "use_graph": 1}),
                                        (3, {"tile_shape": 3, "use_graph": 0}), (8, {"tile_shape": 3, "use_graph": 1}),
                                        (9, {"tile_shape": 3, "use_graph": 1})])
-def test_few_jobs(fz, count, cfg):
+def test_few_jobs(fz, port, count, cfg):
     """Up to eight descriptors over a grid of one workgroup per tile (or the
     grouped shape) travel in the kernel arguments (no descriptor copy); nine,
     or a grid smaller than the tile count, read them from device memory."""
     fz_mod, batch = fz
     specs = make_arrays(30 + count, count=count)
-    ref_s, ref_v = plain_outputs(fz_mod, specs)
+    ref_s, ref_v = oracle_outputs(port, specs)
     outs = [torch.full_like(s, 0x77) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
     vals = [torch.full_like(x, float("nan")) for x, _, _ in specs]
@@ -113,7 +116,7 @@ def test_few_jobs(fz, count, cfg):
 
 @pytest.mark.parametrize("shape", [0, 2])
 @pytest.mark.parametrize("grid", [1, 5, 6, 11, 12, 13, 64])
-def test_round_robin_grid_edges(fz, shape, grid):
+def test_round_robin_grid_edges(fz, port, shape, grid):
     """Static round-robin (no claims) over a tile table: twelve float32 jobs of
     one tile each (more than fit in the kernel arguments), with grids that
     take every tile in one step, leave a remainder, end exactly on a full
@@ -127,7 +130,7 @@ def test_round_robin_grid_edges(fz, shape, grid):
     for i in range(12):
         x = smooth_signal_torch(1000 - 37 * i, seed=500 + i, dtype=torch.float32)
         specs.append((x, (16, 12, 21, 32)[i % 4], 32))
-    ref_s, ref_v = plain_outputs(fz_mod, specs)
+    ref_s, ref_v = oracle_outputs(port, specs)
     cfg = {"tile_shape": shape, "workgroups": grid, "tiles_per_claim": 0, "use_graph": 1}
     outs = [torch.full_like(s, 0x5A) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
@@ -144,10 +147,10 @@ def test_round_robin_grid_edges(fz, shape, grid):
     dplan.close()
 
 
-def test_graph_replays_with_eager_work_between(fz):
+def test_graph_replays_with_eager_work_between(fz, port):
     fz_mod, batch = fz
     specs = make_arrays(2, count=30)
-    ref_s, _ = plain_outputs(fz_mod, specs)
+    ref_s, _ = oracle_outputs(port, specs)
     outs = [torch.empty_like(s) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
     assert plan.config["use_graph"] == 1
@@ -163,10 +166,10 @@ def test_graph_replays_with_eager_work_between(fz):
         del junk
 
 
-def test_autotune_keeps_bytes(fz):
+def test_autotune_keeps_bytes(fz, port):
     fz_mod, batch = fz
     specs = make_arrays(3, count=40)
-    ref_s, _ = plain_outputs(fz_mod, specs)
+    ref_s, _ = oracle_outputs(port, specs)
     outs = [torch.empty_like(s) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
     chosen = plan.autotune(reps=1)
@@ -196,10 +199,10 @@ def test_batch_rejects_bad_jobs(fz):
 
 @pytest.mark.parametrize("cfg", [None, {"tile_shape": 0, "use_graph": 0}, {"tile_shape": 2, "use_graph": 0},
                                  {"tile_shape": 3, "use_graph": 1}], ids=["default", "narrow", "wide", "grouped"])
-def test_decompress_and_gather_of_arbitrary_streams(fz, cfg):
+def test_decompress_and_gather_of_arbitrary_streams(fz, port, cfg):
     """Streams no compress writes (random payload and block exponents over
     the range where the reference decode is defined, E + bias < 0 included):
-    every batched decompress and gather equals the plain call, for each tile
+    every batched decompress and gather equals the oracle's, for each tile
     shape."""
     fz_mod, batch = fz
     from oracle.oracle import compressed_bytes, storage_bits
@@ -230,8 +233,10 @@ def test_decompress_and_gather_of_arbitrary_streams(fz, cfg):
     plan.launch()
     torch.cuda.synchronize()
     for i, (s, o, g, ix, (n, dt, b, B)) in enumerate(zip(streams, outs, gouts, idxs, specs)):
-        want = fz_mod.decompress(s, n, dt, b, B)
-        assert torch.equal(o.view(torch.uint8), want.view(torch.uint8)), f"decompress job {i}: {dt} b{b} B{B} n{n}"
-        wg = fz_mod.gather(s, n, dt, b, B, ix)
-        assert torch.equal(g.view(torch.uint8), wg.view(torch.uint8)), f"gather job {i}"
+        npdt = np.float32 if dt == torch.float32 else np.float64
+        want = port.decompress(s.cpu().numpy(), n, npdt, b, B)
+        assert np.array_equal(o.cpu().numpy().view(np.uint8), want.view(np.uint8)), \
+            f"decompress job {i}: {dt} b{b} B{B} n{n}"
+        wg = port.gather(s.cpu().numpy(), n, npdt, b, B, ix.cpu().numpy().astype(np.uint64))
+        assert np.array_equal(g.cpu().numpy().view(np.uint8), wg.view(np.uint8)), f"gather job {i}"
     plan.close()
