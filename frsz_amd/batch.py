@@ -68,6 +68,20 @@ _L.frsz2_batch_set_config.restype = ctypes.c_int
 _L.frsz2_batch_set_config.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchConfig)]
 _L.frsz2_batch_destroy.restype = ctypes.c_int
 _L.frsz2_batch_destroy.argtypes = [ctypes.c_void_p]
+_L.frsz2_batch_run_noalloc.restype = ctypes.c_int
+_L.frsz2_batch_run_noalloc.argtypes = [ctypes.POINTER(Job), ctypes.c_size_t, ctypes.c_void_p]
+
+NOALLOC_MAX_JOBS = 10922
+
+
+def run_noalloc(jobs, stream: int | None = None) -> None:
+    """Run ``jobs`` (from compress_job / decompress_job / gather_job) once, as
+    a plan with the grouped shape would, without allocating device memory
+    (frsz2_batch_run_noalloc).  Calls on one device that overlap in time
+    share its descriptor scratch and race; calls on one stream do not."""
+    arr = (Job * len(jobs))(*[j for j, _ in jobs])
+    s = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    check(_L.frsz2_batch_run_noalloc(arr, len(jobs), s))
 
 
 def _require(t: torch.Tensor, what: str, dtypes=None) -> None:
@@ -161,5 +175,5 @@ class BatchPlan:
             pass
 
 
-__all__ = ["BatchPlan", "compress_job", "decompress_job", "gather_job", "compressed_size", "OP_COMPRESS",
+__all__ = ["BatchPlan", "run_noalloc", "NOALLOC_MAX_JOBS", "compress_job", "decompress_job", "gather_job", "compressed_size", "OP_COMPRESS",
            "OP_DECOMPRESS", "OP_GATHER"]

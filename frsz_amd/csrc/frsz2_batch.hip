@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -480,6 +481,29 @@ aligned16(const void* p)
 
 // grouped mode: one k_grouped launch per run of same-class descriptors
 // (descriptors are built in class order)
+// the grouped kernel of class cls over descriptors d[0..nd) (h: their host
+// copy when they go in the kernel arguments), tiles [t0, t0 + nt)
+hipError_t
+launch_class_of(int cls, const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64_t nt,
+                const std::uint32_t* jtab, hipStream_t s)
+{
+#define FRSZ2_G(K)                                                                                           \
+  case K:                                                                                                    \
+    launch_class<K>(d, h, nd, t0, nt, jtab, s);                                                              \
+    break
+  switch (cls) {
+    FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
+    FRSZ2_G(7); FRSZ2_G(8); FRSZ2_G(9); FRSZ2_G(10); FRSZ2_G(11); FRSZ2_G(12); FRSZ2_G(13);
+    FRSZ2_G(14); FRSZ2_G(15); FRSZ2_G(16); FRSZ2_G(17); FRSZ2_G(18); FRSZ2_G(19); FRSZ2_G(20);
+    FRSZ2_G(21); FRSZ2_G(22); FRSZ2_G(23); FRSZ2_G(24); FRSZ2_G(25); FRSZ2_G(26); FRSZ2_G(27);
+    FRSZ2_G(28); FRSZ2_G(29); FRSZ2_G(30); FRSZ2_G(31); FRSZ2_G(32); FRSZ2_G(33); FRSZ2_G(34);
+    default:
+      break;
+  }
+#undef FRSZ2_G
+  return hipGetLastError();
+}
+
 hipError_t
 launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, const std::uint32_t* jtab,
                bool inl, hipStream_t s)
@@ -494,21 +518,8 @@ launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t tot
     const std::uint64_t t0 = h_desc[d0].tile0;
     const std::uint64_t t1 = d1 < nd ? h_desc[d1].tile0 : total_tiles;
     if (t1 > t0) {
-#define FRSZ2_G(K)                                                                                           \
-  case K:                                                                                                    \
-    launch_class<K>(d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s);                \
-    break
-      switch (cls) {
-        FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
-        FRSZ2_G(7); FRSZ2_G(8); FRSZ2_G(9); FRSZ2_G(10); FRSZ2_G(11); FRSZ2_G(12); FRSZ2_G(13);
-        FRSZ2_G(14); FRSZ2_G(15); FRSZ2_G(16); FRSZ2_G(17); FRSZ2_G(18); FRSZ2_G(19); FRSZ2_G(20);
-        FRSZ2_G(21); FRSZ2_G(22); FRSZ2_G(23); FRSZ2_G(24); FRSZ2_G(25); FRSZ2_G(26); FRSZ2_G(27);
-        FRSZ2_G(28); FRSZ2_G(29); FRSZ2_G(30); FRSZ2_G(31); FRSZ2_G(32); FRSZ2_G(33); FRSZ2_G(34);
-        default:
-          break;
-      }
-#undef FRSZ2_G
-      const hipError_t e = hipGetLastError();
+      const hipError_t e =
+        launch_class_of(cls, d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s);
       if (e != hipSuccess) {
         return e;
       }
@@ -560,6 +571,46 @@ validate_job(const frsz2_job& j, int idx)
   return FRSZ2_OK;
 }
 
+// Descriptor of job j for the tile path at tile_shape, its tiles starting at
+// global tile tile0: the job's tile count, or -1 when the tile path cannot
+// take it (B not a power of two or larger than the class's tile, pointers
+// off 16-byte alignment), in which case it runs as a plain launch.
+std::int64_t
+describe_job(const frsz2_job& j, int tile_shape, std::uint64_t tile0, bjob* d)
+{
+  const int cls = class_of(j.op, j.dtype, j.bits);
+  const layout L = layout::make(j.bits, j.block, j.n);
+  const int lb = ilog2_exact(j.block);
+  std::uint64_t nt;
+  if (j.op == FRSZ2_OP_GATHER) {
+    nt = (j.m + GATHER_TILE - 1) / GATHER_TILE;
+  } else {
+    const std::uint64_t TILE = static_cast<std::uint64_t>(class_wgs(cls, tile_shape)) * class_vpt(cls, tile_shape);
+    if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
+      return -1;
+    }
+    nt = (j.n + TILE - 1) / TILE;
+    if (j.op == FRSZ2_OP_COMPRESS && nt == 0) {
+      nt = 1; // the pad of an empty stream is still written
+    }
+  }
+  *d = bjob{};
+  d->in = j.in;
+  d->out = j.out;
+  d->idx = j.idx;
+  d->n = j.n;
+  d->nb = L.nb;
+  d->R = L.R;
+  d->tile0 = tile0;
+  d->m = j.m;
+  d->B = j.block;
+  d->CB = static_cast<std::uint32_t>(L.CB);
+  d->bits = j.bits;
+  d->lb = lb;
+  d->cls = cls;
+  return static_cast<std::int64_t>(nt);
+}
+
 std::uint64_t
 build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vector<bjob>& desc,
                   std::vector<char>& tiled)
@@ -576,44 +627,17 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
     return class_of(jobs[a].op, jobs[a].dtype, jobs[a].bits) < class_of(jobs[b].op, jobs[b].dtype, jobs[b].bits);
   });
   for (const std::size_t i : order) {
-    const frsz2_job& j = jobs[i];
-    const int cls = class_of(j.op, j.dtype, j.bits);
-    const layout L = layout::make(j.bits, j.block, j.n);
-    const int lb = ilog2_exact(j.block);
-    std::uint64_t nt;
-    if (j.op == FRSZ2_OP_GATHER) {
-      nt = (j.m + GATHER_TILE - 1) / GATHER_TILE;
-    } else {
-      const std::uint64_t TILE =
-        static_cast<std::uint64_t>(class_wgs(cls, tile_shape)) * class_vpt(cls, tile_shape);
-      if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
-        continue;
-      }
-      nt = (j.n + TILE - 1) / TILE;
-      if (j.op == FRSZ2_OP_COMPRESS && nt == 0) {
-        nt = 1; // the pad of an empty stream is still written
-      }
+    bjob d;
+    const std::int64_t nt = describe_job(jobs[i], tile_shape, tiles, &d);
+    if (nt < 0) {
+      continue;
     }
     tiled[i] = 1;
     if (nt == 0) {
       continue;
     }
-    bjob d{};
-    d.in = j.in;
-    d.out = j.out;
-    d.idx = j.idx;
-    d.n = j.n;
-    d.nb = L.nb;
-    d.R = L.R;
-    d.tile0 = tiles;
-    d.m = j.m;
-    d.B = j.block;
-    d.CB = static_cast<std::uint32_t>(L.CB);
-    d.bits = j.bits;
-    d.lb = lb;
-    d.cls = cls;
     desc.push_back(d);
-    tiles += nt;
+    tiles += static_cast<std::uint64_t>(nt);
   }
   return tiles;
 }
@@ -1072,6 +1096,177 @@ frsz2_batch_destroy(frsz2_batch_plan* plan)
   frsz2::clear_error();
   if (plan) {
     frsz2::destroy(plan);
+  }
+  return FRSZ2_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- allocation-free batched launch
+//
+// frsz2_batch_run_noalloc runs a batch the way a plan with the grouped shape
+// does (one launch per kernel class present, one workgroup per tile, the
+// plain kernels' variants), without allocating: the descriptors of the
+// tiled jobs live in `scratch`, a file-scope device array of
+// NOALLOC_SCRATCH_BYTES that every call on a device shares; a workgroup
+// finds its tile's job by binary search over its class's descriptors (no
+// tile table).  The descriptors are written into scratch by small
+// stream-ordered kernels that take them in their kernel arguments, so no
+// host staging buffer is involved.  Jobs the tiles cannot take run as plain
+// launches, as in a plan.
+
+namespace frsz2 {
+namespace {
+
+constexpr std::size_t NOALLOC_SCRATCH_BYTES = std::size_t{ 1 } << 20;
+constexpr std::size_t NOALLOC_MAX_DESC = NOALLOC_SCRATCH_BYTES / sizeof(bjob); // 10922
+
+// descriptors per store kernel: 40 x 96 bytes of kernel arguments
+constexpr int STORE_JOBS = 40;
+struct store_pack
+{
+  bjob j[STORE_JOBS];
+};
+static_assert(sizeof(store_pack) + 16 <= 4096, "store kernel arguments fit the kernel argument space");
+
+// The device copy of the array is only reachable through the address
+// hipGetSymbolAddress returns for the current device; its name in host code
+// is not a device address.
+static __device__ __align__(256) std::uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+// dst[i] = pack.j[i] for i < count (one thread per descriptor)
+__global__ __launch_bounds__(64) void k_store_descriptors(store_pack pack, int count, bjob* __restrict__ dst)
+{
+  const int i = static_cast<int>(threadIdx.x);
+  if (i < count) {
+    dst[i] = pack.j[i];
+  }
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[MAX_DEVICES] = {};
+
+// device address of the current device's copy of `scratch`, looked up once
+// per device with that device current
+int
+scratch_address(bjob** out)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    return hip_status(e, "hipGetDevice");
+  }
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    return set_error(FRSZ2_EARG, "device " + std::to_string(dev) + " beyond the allocation-free path's " +
+                                   std::to_string(MAX_DEVICES) + " devices");
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      return hip_status(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    }
+    g_scratch[dev] = p;
+  }
+  *out = static_cast<bjob*>(g_scratch[dev]);
+  return FRSZ2_OK;
+}
+
+constexpr int NUM_CLASSES = 35;
+
+} // namespace
+} // namespace frsz2
+
+extern "C" {
+
+int
+frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t stream)
+{
+  using namespace frsz2;
+  clear_error();
+  if (njobs > 0 && !jobs) {
+    return set_error(FRSZ2_EARG, "null jobs");
+  }
+  for (size_t i = 0; i < njobs; ++i) {
+    const int rc = validate_job(jobs[i], static_cast<int>(i));
+    if (rc) {
+      return rc;
+    }
+  }
+  // descriptors needed: one per job the tiles take that has tiles
+  std::size_t ndesc = 0;
+  for (size_t i = 0; i < njobs; ++i) {
+    bjob d;
+    if (describe_job(jobs[i], 3, 0, &d) > 0) {
+      ++ndesc;
+    }
+  }
+  if (ndesc > NOALLOC_MAX_DESC) {
+    return set_error(FRSZ2_ECAPACITY, "the allocation-free batch takes at most " + std::to_string(NOALLOC_MAX_DESC) +
+                                        " tiled jobs (" + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                        " bytes of descriptor scratch); this batch has " + std::to_string(ndesc));
+  }
+  const auto s = static_cast<hipStream_t>(stream);
+  bjob* d_scratch = nullptr;
+  if (ndesc > 0) {
+    const int rc = scratch_address(&d_scratch);
+    if (rc) {
+      return rc;
+    }
+  }
+  // class by class, in job order within a class (the order a plan uses):
+  // store the class's descriptors, then launch its grouped kernel over them
+  std::size_t pos = 0;     // descriptors stored so far
+  std::uint64_t tiles = 0; // global tile numbering, as in a plan
+  store_pack pack{};
+  for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+    const std::size_t d0 = pos;
+    const std::uint64_t t0 = tiles;
+    int fill = 0;
+    std::size_t flushed = pos;
+    for (size_t i = 0; i < njobs; ++i) {
+      const frsz2_job& j = jobs[i];
+      if (class_of(j.op, j.dtype, j.bits) != cls) {
+        continue;
+      }
+      bjob d;
+      const std::int64_t nt = describe_job(j, 3, tiles, &d);
+      if (nt <= 0) {
+        continue;
+      }
+      pack.j[fill++] = d;
+      ++pos;
+      tiles += static_cast<std::uint64_t>(nt);
+      if (fill == STORE_JOBS) {
+        hipLaunchKernelGGL(k_store_descriptors, dim3(1), dim3(64), 0, s, pack, fill, d_scratch + flushed);
+        flushed += static_cast<std::size_t>(fill);
+        fill = 0;
+      }
+    }
+    if (fill > 0) {
+      hipLaunchKernelGGL(k_store_descriptors, dim3(1), dim3(64), 0, s, pack, fill, d_scratch + flushed);
+    }
+    if (pos > d0) {
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) {
+        e = launch_class_of(cls, d_scratch + d0, nullptr, static_cast<int>(pos - d0), t0, tiles - t0, nullptr, s);
+      }
+      if (e != hipSuccess) {
+        return hip_status(e, "allocation-free batch launch");
+      }
+    }
+  }
+  // jobs outside the tile path: plain launches on the same stream
+  for (size_t i = 0; i < njobs; ++i) {
+    bjob d;
+    if (describe_job(jobs[i], 3, 0, &d) < 0) {
+      const int rc = run_plain(jobs[i], s);
+      if (rc) {
+        return rc;
+      }
+    }
   }
   return FRSZ2_OK;
 }

@@ -281,6 +281,22 @@ int frsz2_batch_get_config(const frsz2_batch_plan* plan, frsz2_batch_config* cfg
 int frsz2_batch_set_config(frsz2_batch_plan* plan, const frsz2_batch_config* cfg);
 int frsz2_batch_destroy(frsz2_batch_plan* plan);
 
+/* Run the jobs once, as frsz2_batch_create(jobs, njobs, NULL) +
+ * frsz2_batch_launch + frsz2_batch_destroy would (grouped launches, same
+ * bytes), without allocating any memory: the descriptors of the jobs the
+ * tiled kernels take go to a 1 MiB file-scope device array of the library
+ * (at most FRSZ2_NOALLOC_MAX_JOBS such jobs; more fail with
+ * FRSZ2_ECAPACITY before anything runs), written by stream-ordered kernels
+ * on `stream`.  No counterpart in the reference: this is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Every call on a device shares
+ * that device's array, so calls on one device whose launches overlap in time
+ * (other streams, other threads) race; calls ordered on one stream do not.
+ * The array stays allocated on each device the library has run this call
+ * on until the process ends. */
+#define FRSZ2_NOALLOC_MAX_JOBS 10922u
+int frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t stream);
+
 /* ---- serving: resident streams, HBM-budget admission, continuous batching ----
  * No counterpart in the reference (its API is one synchronous call per
  * array).  A server owns a device pool of hbm_budget bytes from which every
