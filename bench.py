@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-copy", action="store_true", help="skip the 1 GiB device-copy calibration")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-edge", action="store_true")
+    ap.add_argument("--no-generic", action="store_true", help="skip the non-power-of-two block sizes")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the float32 2^20-value case")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -206,6 +207,77 @@ def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
                                  "matches_decompress": bool(torch.equal(go.view(torch.int32), ref.view(torch.int32)))}
             del s, first
     del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
+def _best_time(fn, reps):
+    """Seconds per call of fn: best of 3 rounds of `reps` calls, after 2
+    untimed ones, timed with events on torch's current stream (the stream
+    the codec launches on)."""
+    import torch
+
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / reps * 1e-3
+        best = t if best is None else min(best, t)
+    return best
+
+
+def bench_generic(n=100_000_037, reps=5):
+    """Block sizes that are not powers of two (the generic tiles) and
+    streams whose payload is off 16-byte alignment, on the edge-case data
+    (float32, n = 100,000,037; float64 cases on its first 5*10^7 values
+    widened): compress / decompress TB/s of algorithmic bytes (values +
+    stream).  Every output is compared with the first launch's."""
+    import torch
+
+    import frsz_amd
+    from frsz_amd.workloads import lognormal_edge
+
+    x32 = torch.from_numpy(lognormal_edge(n, seed=3)).cuda()
+    x64 = x32[: 50_000_000].double()
+    out = {"n_float32": n, "n_float64": x64.numel(), "cases": []}
+
+    def run(x, bits, B, offset=0):
+        n_ = x.numel()
+        size = frsz_amd.compressed_size(x.dtype, bits, B, n_)
+        buf = torch.zeros(size + offset, dtype=torch.uint8, device="cuda")
+        s = buf[offset:]
+        y = torch.empty_like(x)
+        frsz_amd.compress_into(x, s, bits, B)
+        first = s.clone()
+        tc = _best_time(lambda: frsz_amd.compress_into(x, s, bits, B), reps)
+        same = torch.equal(s, first)
+        frsz_amd.decompress_into(s, y, bits, B)
+        y0 = y.clone()
+        td = _best_time(lambda: frsz_amd.decompress_into(s, y, bits, B), reps)
+        same = same and torch.equal(y.view(torch.uint8), y0.view(torch.uint8))
+        alg = n_ * x.element_size() + size
+        out["cases"].append({"dtype": str(x.dtype).replace("torch.", ""), "bits": bits, "block": B,
+                             "stream_offset": offset, "compress_tbs": round(alg / tc / 1e12, 3),
+                             "decompress_tbs": round(alg / td / 1e12, 3), "deterministic": bool(same)})
+        del buf, s, y, first, y0
+
+    for B in (3, 5, 7, 9, 12, 100):
+        for bits in (8, 12, 16, 21, 32):
+            run(x32, bits, B)
+    for B in (5, 100):
+        for bits in (12, 21, 40):
+            run(x64, bits, B)
+    # payload 8 bytes off 16-byte alignment: power-of-two B on the generic tiles
+    for bits in (12, 21, 32):
+        run(x32, bits, 32, offset=8)
+    del x32, x64
     torch.cuda.empty_cache()
     return out
 
@@ -696,6 +768,9 @@ def main():
     if not args.no_edge:
         print("bench: edge-case sweep", file=sys.stderr, flush=True)
         line["edge"] = bench_edge()
+    if not args.no_generic:
+        print("bench: block sizes that are not powers of two, unaligned streams", file=sys.stderr, flush=True)
+        line["generic"] = bench_generic()
     if not args.no_serve:
         line["serve"] = bench_serve(args.serve_seconds, args.serve_rate, args.serve_budget_gib)
     if not args.no_verify or not args.no_cpu:

@@ -14,6 +14,7 @@
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
+#include "gtile.hpp"
 #include "tile.hpp"
 
 #include <hip/hip_runtime.h>
@@ -197,6 +198,28 @@ __global__ __launch_bounds__(WG) void k_encode_words(const T* __restrict__ in,
   reinterpret_cast<wt*>(out + L.R + k * L.CB)[w] = static_cast<wt>(acc);
 }
 
+// Generic-B tiles (gtile.hpp): one workgroup per tile of TB whole blocks;
+// `align` is the alignment of the payload (16, 8, 4 or 1 bytes)
+template<typename T, int U, int VPT, bool PACKED>
+__global__ __launch_bounds__(gtile::WG) void k_compress_gtile(const T* __restrict__ in,
+                                                              std::uint8_t* __restrict__ out,
+                                                              gtile::gplan p,
+                                                              int align)
+{
+  extern __shared__ std::uint32_t gsmem[];
+  gtile::compress_gtile<T, U, VPT, PACKED, true>(in, out, p, blockIdx.x, gsmem, align);
+}
+
+template<typename T, int U, int VPT, bool PACKED>
+__global__ __launch_bounds__(gtile::WG) void k_decompress_gtile(const std::uint8_t* __restrict__ in,
+                                                                T* __restrict__ out,
+                                                                gtile::gplan p,
+                                                                int align)
+{
+  extern __shared__ std::uint32_t gsmem[];
+  gtile::decompress_gtile<T, U, VPT, PACKED, true>(in, out, p, blockIdx.x, gsmem, align);
+}
+
 template<typename T, int U>
 __global__ __launch_bounds__(WG) void k_decompress_elem(const std::uint8_t* __restrict__ in,
                                                         T* __restrict__ out,
@@ -290,10 +313,110 @@ launch_decompress_tiled(const void* in, void* out, const layout& L, int lb, int 
   return hipGetLastError();
 }
 
+int
+alignment_of(const void* p)
+{
+  const std::uintptr_t a = reinterpret_cast<std::uintptr_t>(p);
+  return (a & 15u) == 0 ? 16 : (a & 7u) == 0 ? 8 : (a & 3u) == 0 ? 4 : 1;
+}
+
+std::uint32_t
+gcd32(std::uint32_t a, std::uint32_t b)
+{
+  while (b != 0) {
+    const std::uint32_t r = a % b;
+    a = b;
+    b = r;
+  }
+  return a;
+}
+
+// Generic-tile plan of a run, or false when the tiles cannot take it (no
+// values, B = 1, or a block whose aligned tile would exceed GT_MAX_VALUES).  TB is
+// the largest multiple of 16 / gcd(CB, 16) blocks within TARGET_VALUES
+// values (at least that multiple), so every tile's payload is a whole
+// number of 16-byte chunks.
+bool
+make_gplan(const layout& L, std::uint32_t zpad, gtile::gplan* g)
+{
+  if (L.n == 0 || L.B < 2 || L.B > static_cast<std::uint64_t>(gtile::GT_MAX_VALUES)) {
+    return false;
+  }
+  const std::uint32_t B = static_cast<std::uint32_t>(L.B);
+  const std::uint32_t CB = static_cast<std::uint32_t>(L.CB);
+  const std::uint32_t step = 16u / gcd32(CB, 16u);
+  std::uint32_t TB = static_cast<std::uint32_t>(gtile::TARGET_VALUES) / B / step * step;
+  if (TB == 0) {
+    TB = step;
+  }
+  if (static_cast<std::uint64_t>(TB) * B > static_cast<std::uint64_t>(gtile::GT_MAX_VALUES)) {
+    return false;
+  }
+  *g = gtile::gplan{};
+  g->n = L.n;
+  g->nb = L.nb;
+  g->R = L.R;
+  g->B = B;
+  g->CB = CB;
+  g->TB = TB;
+  g->V = TB * B;
+  g->magic = static_cast<std::uint32_t>(((std::uint64_t{ 1 } << 32) + B - 1) / B);
+  // lanes per block for the block maxima: as many as keep all of the tile's
+  // blocks in one pass of the workgroup (at most 16, at most B)
+  g->glg = 0;
+  while (g->glg < 4 && (2u << g->glg) <= B && TB * (2u << g->glg) <= static_cast<std::uint32_t>(gtile::WG)) {
+    ++g->glg;
+  }
+  g->bits = L.bits;
+  g->zpad = zpad;
+  return true;
+}
+
+template<typename T, int U, int VPT, bool PACKED>
+hipError_t
+launch_gtile_v(bool compress, const void* in, void* out, const gtile::gplan& g, hipStream_t s)
+{
+  const std::uint64_t tiles = (g.n + g.V - 1) / g.V;
+  const std::size_t lds =
+    static_cast<std::size_t>(compress ? gtile::compress_lds_dwords(g) : gtile::image_dwords(g) + g.TB) * 4u;
+  const dim3 grid(static_cast<unsigned>(tiles));
+  if (compress) {
+    const int align = alignment_of(static_cast<const std::uint8_t*>(out) + g.R);
+    hipLaunchKernelGGL((k_compress_gtile<T, U, VPT, PACKED>), grid, dim3(gtile::WG), lds, s,
+                       static_cast<const T*>(in), static_cast<std::uint8_t*>(out), g, align);
+  } else {
+    const int align = alignment_of(static_cast<const std::uint8_t*>(in) + g.R);
+    hipLaunchKernelGGL((k_decompress_gtile<T, U, VPT, PACKED>), grid, dim3(gtile::WG), lds, s,
+                       static_cast<const std::uint8_t*>(in), static_cast<T*>(out), g, align);
+  }
+  return hipGetLastError();
+}
+
+// tiles of more than TARGET_VALUES values take 16 values per thread
+template<typename T, int U>
+hipError_t
+launch_gtile(bool compress, const void* in, void* out, const gtile::gplan& g, hipStream_t s)
+{
+  constexpr int V8 = gtile::TARGET_VALUES / gtile::WG;
+  constexpr int V16 = gtile::GT_MAX_VALUES / gtile::WG;
+  const bool wide = g.V > static_cast<std::uint32_t>(gtile::TARGET_VALUES);
+  const bool packed = g.bits < U;
+  if (wide) {
+    return packed ? launch_gtile_v<T, U, V16, true>(compress, in, out, g, s)
+                  : launch_gtile_v<T, U, V16, false>(compress, in, out, g, s);
+  }
+  return packed ? launch_gtile_v<T, U, V8, true>(compress, in, out, g, s)
+                : launch_gtile_v<T, U, V8, false>(compress, in, out, g, s);
+}
+
 template<typename T, int U>
 hipError_t
 launch_compress_generic(const void* in, void* out, const layout& L, std::uint32_t zpad, hipStream_t s)
 {
+  gtile::gplan g;
+  if (make_gplan(L, zpad, &g)) {
+    return launch_gtile<T, U>(true, in, out, g, s);
+  }
   if (L.B <= GENERIC_SMALL_B) {
     const std::uint64_t threads = L.nb < 256 ? 256 : L.nb;
     hipLaunchKernelGGL((k_compress_block<T, U>), dim3(grid_for(threads)), dim3(WG), 0, s, static_cast<const T*>(in),
@@ -326,6 +449,10 @@ launch_decompress_generic(const void* in, void* out, const layout& L, hipStream_
 {
   if (L.n == 0) {
     return hipSuccess;
+  }
+  gtile::gplan g;
+  if (make_gplan(L, 0, &g)) {
+    return launch_gtile<T, U>(false, in, out, g, s);
   }
   hipLaunchKernelGGL((k_decompress_elem<T, U>), dim3(grid_for(L.n)), dim3(WG), 0, s,
                      static_cast<const std::uint8_t*>(in), static_cast<T*>(out), L);

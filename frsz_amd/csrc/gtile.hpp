@@ -1,0 +1,318 @@
+// Tiles of whole exponent blocks for any block size B (gfx950): the path for
+// block sizes that are not powers of two, and for payloads off 16-byte
+// alignment, where the power-of-two tiles (tile.hpp) do not apply.
+//
+// A tile is TB whole exponent blocks, V = TB*B values (at most GT_MAX_VALUES),
+// with TB chosen on the host so that TB*CB is a multiple of 16 bytes: every
+// tile's payload then starts at the same alignment as the stream's payload.
+// Thread t handles tile values t, t + WG, ... (consecutive lanes on
+// consecutive values, so every wave load / store is one contiguous span).
+// The block of tile value i is i / B, by one multiply-high with a magic
+// number (exact for i < 2^16, 2 <= B <= 2^16).  Lanes past the tile's values
+// (V is often below VPT*WG; the last tile is short) take the tile's last
+// value for their loads and arithmetic, so the per-value code has no
+// branches, and only their LDS atomics and stores are masked off.
+//
+// compress:   load -> block maxima (LDS atomics for B < 8; otherwise keys
+//             into LDS and one block per group of up to 16 lanes, a DPP
+//             butterfly) -> block exponents ->
+//             codes OR-ed into an LDS image of the tile's payload at their bit
+//             offsets -> exponents and the image written out (the widest
+//             stores the payload alignment allows).  Same bytes as
+//             compress_cpu_block (include/frsz2.hpp:1108-1179).
+// decompress: the tile's payload staged into LDS with wide loads, its block
+//             exponents into LDS, then per value: extract the code, decode,
+//             store.  Same values as decompress_cpu_block (:1217-1240).
+#pragma once
+
+#include "frsz2_device.hpp"
+#include "tile.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace frsz2 {
+namespace gtile {
+
+constexpr int WG = 256;
+constexpr int GT_MAX_VALUES = 4096;
+// tiles of up to TARGET_VALUES values take VPT = TARGET_VALUES / WG values
+// per thread, larger ones (TB rounded up for alignment) GT_MAX_VALUES / WG
+constexpr int TARGET_VALUES = 2048;
+
+// per-run parameters (host-computed)
+struct gplan
+{
+  std::uint64_t n;      // values of the run
+  std::uint64_t nb;     // exponent blocks of the run
+  std::uint64_t R;      // payload offset from the run's exponent base
+  std::uint32_t B;      // block size
+  std::uint32_t CB;     // payload bytes per block
+  std::uint32_t TB;     // blocks per tile
+  std::uint32_t V;      // values per tile (TB * B)
+  std::uint32_t magic;  // ceil(2^32 / B), B >= 2
+  int glg;              // log2 of the lanes that reduce one block's maximum (2^glg <= min(16, B))
+  int bits;
+  std::uint32_t zpad;   // exponent-pad bytes after slot nb to clear (tile 0)
+};
+
+// block of tile value i (i < 2^16): one multiply-high, exact for every
+// B >= 2 (for powers of two the magic number is exactly 2^32 / B)
+__device__ __forceinline__ std::uint32_t
+block_of(std::uint32_t i, const gplan& p)
+{
+  return __umulhi(i, p.magic);
+}
+
+// LDS dwords of a tile's payload image, plus slack for the reads and ORs
+// that touch the dwords after a code's last one
+__host__ __device__ inline std::uint32_t
+image_dwords(const gplan& p)
+{
+  return p.TB * p.CB / 4 + 4;
+}
+
+// copy nbytes between global memory at g (aligned like the stream payload:
+// `align` = 16, 8 or 4 bytes, or 1) and the LDS image, all threads of the
+// workgroup taking part
+template<bool TO_LDS, bool NT>
+__device__ __forceinline__ void
+move_payload(std::uint8_t* g, std::uint32_t* lds, std::uint32_t nbytes, int align)
+{
+  using tile::u32x4;
+  typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  const std::uint32_t t = threadIdx.x;
+  std::uint32_t done = 0;
+  if (align >= 16) {
+    u32x4* g4 = reinterpret_cast<u32x4*>(g);
+    u32x4* l4 = reinterpret_cast<u32x4*>(lds);
+    for (std::uint32_t c = t; c < nbytes / 16; c += WG) {
+      if constexpr (TO_LDS) {
+        l4[c] = tile::ld<NT>(g4 + c);
+      } else {
+        tile::st<NT>(l4[c], g4 + c);
+      }
+    }
+    done = nbytes & ~15u;
+  } else if (align >= 8) {
+    u32x2* g2 = reinterpret_cast<u32x2*>(g);
+    u32x2* l2 = reinterpret_cast<u32x2*>(lds);
+    for (std::uint32_t c = t; c < nbytes / 8; c += WG) {
+      if constexpr (TO_LDS) {
+        l2[c] = tile::ld<NT>(g2 + c);
+      } else {
+        tile::st<NT>(l2[c], g2 + c);
+      }
+    }
+    done = nbytes & ~7u;
+  } else if (align >= 4) {
+    std::uint32_t* g1 = reinterpret_cast<std::uint32_t*>(g);
+    for (std::uint32_t c = t; c < nbytes / 4; c += WG) {
+      if constexpr (TO_LDS) {
+        lds[c] = g1[c];
+      } else {
+        g1[c] = lds[c];
+      }
+    }
+    done = nbytes & ~3u;
+  }
+  std::uint8_t* lb = reinterpret_cast<std::uint8_t*>(lds);
+  for (std::uint32_t o = done + t; o < nbytes; o += WG) {
+    if constexpr (TO_LDS) {
+      lb[o] = g[o];
+    } else {
+      g[o] = lb[o];
+    }
+  }
+}
+
+// LDS dwords of a compress tile: payload image, TB block keys, V 16-bit value keys
+__host__ __device__ inline std::uint32_t
+compress_lds_dwords(const gplan& p)
+{
+  return image_dwords(p) + p.TB + (p.V + 1) / 2;
+}
+
+// Compress tile `tile` of a run: exponents at out, payload at out + p.R.
+// LDS: compress_lds_dwords(p).  PACKED: bits < U (codes OR-ed into the
+// image at their bit offsets); otherwise the image is an array of U-bit
+// words, word i holding value i's code (CB = B*U/8 then, no block padding).
+template<typename T, int U, int VPT, bool PACKED, bool NT>
+__device__ __forceinline__ void
+compress_gtile(const T* __restrict__ in, std::uint8_t* __restrict__ out, const gplan& p, std::uint64_t tile,
+               std::uint32_t* smem, int align)
+{
+  using uw = typename fmt<T>::uw;
+  using wt = typename word<U>::type;
+  const std::uint32_t t = threadIdx.x;
+  std::uint32_t* img = smem;
+  const std::uint32_t idw = image_dwords(p);
+  std::uint32_t* keys = smem + idw;
+  const std::uint64_t k0 = tile * p.TB;
+  const std::uint64_t base = k0 * p.B;
+  const std::uint64_t left = p.n - base;
+  const std::uint32_t cnt = left < p.V ? static_cast<std::uint32_t>(left) : p.V;
+  const std::uint32_t nblk = block_of(cnt - 1, p) + 1;
+  for (std::uint32_t i = t; i < idw; i += WG) {
+    img[i] = 0;
+  }
+  const T* src = in + base;
+  uw x[VPT];
+  std::uint32_t ic[VPT]; // the value index a lane works on (clamped to the tile's last)
+#pragma unroll
+  for (int r = 0; r < VPT; ++r) {
+    const std::uint32_t i = t + static_cast<std::uint32_t>(r * WG);
+    ic[r] = i < cnt ? i : cnt - 1;
+    x[r] = tile::ld<NT>(reinterpret_cast<const uw*>(src) + ic[r]);
+  }
+  // Block maxima into keys.  A wave's 64 values are consecutive.  For B < 8
+  // its lanes spread over many keys, and each lane atomically max-es its key
+  // in.  For larger B, where many lanes would hit one address and
+  // same-address LDS atomics serialize, every value's key goes to an LDS
+  // array (16 bits each, consecutive lanes on consecutive entries), and
+  // aligned groups of G = 2^glg lanes (G <= 16, at most B, chosen so that
+  // one pass of the workgroup covers the tile's blocks where it can) take
+  // one block each: lane l of a group reads entries l, l + G, ... of its
+  // block, and a DPP butterfly within the group gives the block maximum.
+  if (p.B < 8) {
+    for (std::uint32_t i = t; i < nblk; i += WG) {
+      keys[i] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < VPT; ++r) {
+      if (t + static_cast<std::uint32_t>(r * WG) < cnt) {
+        atomicMax(&keys[block_of(ic[r], p)], max_key<T>(x[r]));
+      }
+    }
+  } else {
+    std::uint16_t* kv = reinterpret_cast<std::uint16_t*>(keys + p.TB);
+#pragma unroll
+    for (int r = 0; r < VPT; ++r) {
+      if (t + static_cast<std::uint32_t>(r * WG) < cnt) {
+        kv[ic[r]] = static_cast<std::uint16_t>(max_key<T>(x[r]));
+      }
+    }
+    __syncthreads();
+    const int G = 1 << p.glg;
+    const std::uint32_t l = t & static_cast<std::uint32_t>(G - 1);
+    const std::uint32_t groups = static_cast<std::uint32_t>(WG) >> p.glg;
+    for (std::uint32_t b0 = 0; b0 < nblk; b0 += groups) {
+      const std::uint32_t b = b0 + (t >> p.glg);
+      std::uint32_t m = 0;
+      if (b < nblk) {
+        const std::uint32_t first = b * p.B;
+        const std::uint32_t end = first + p.B < cnt ? first + p.B : cnt;
+        for (std::uint32_t j = first + l; j < end; j += static_cast<std::uint32_t>(G)) {
+          const std::uint32_t v = kv[j];
+          m = m > v ? m : v;
+        }
+      }
+      m = tile::group_max(m, G, nullptr); // G <= 16: DPP steps within the group only
+      if (l == 0 && b < nblk) {
+        keys[b] = m;
+      }
+    }
+  }
+  __syncthreads();
+  const bool narrow = p.bits <= fmt<T>::SB + 1;
+  const std::uint32_t bits = static_cast<std::uint32_t>(p.bits);
+  // bit offset of tile value i in block k = i / B: i*bits plus the padding
+  // bits of the k blocks before it, k*(CB*8 - B*bits)
+  const std::uint32_t padbits = p.CB * 8u - p.B * bits;
+#pragma unroll
+  for (int r = 0; r < VPT; ++r) {
+    const std::uint32_t i = ic[r];
+    const bool own = t + static_cast<std::uint32_t>(r * WG) < cnt;
+    const std::uint32_t k = block_of(i, p);
+    const int E = block_exponent_from_key<T, U>(keys[k]);
+    const uw code = narrow ? encode_value_short<T>(x[r], E, p.bits) : encode_value<T>(x[r], E, p.bits);
+    if constexpr (PACKED) {
+      // a zero code leaves the zeroed image as it is; skipping it saves the
+      // LDS atomics (codes below the block's top bits are often zero)
+      if (own && code != 0) {
+        tile::or_code<U>(img, i * bits + k * padbits, static_cast<std::uint64_t>(code), p.bits);
+      }
+    } else if (own) {
+      reinterpret_cast<wt*>(img)[i] = static_cast<wt>(code);
+    }
+  }
+  for (std::uint32_t b = t; b < nblk; b += WG) {
+    store_block_exponent<U>(out, k0 + b, block_exponent_from_key<T, U>(keys[b]));
+  }
+  __syncthreads();
+  move_payload<false, NT>(out + p.R + k0 * p.CB, img, nblk * p.CB, align);
+  if (tile == 0) {
+    tile::zero_plan_pad<U>(out, tile::tplan{ p.n, p.nb, p.R, p.CB, p.bits, 0, p.zpad }, static_cast<int>(t), WG);
+  }
+}
+
+// Decompress tile `tile` of a run (exponents at in, payload at in + p.R)
+// into out.  LDS: image_dwords(p) dwords of payload, then TB exponents.
+// PACKED: bits < U; otherwise word i of the image is value i's code.
+template<typename T, int U, int VPT, bool PACKED, bool NT>
+__device__ __forceinline__ void
+decompress_gtile(const std::uint8_t* __restrict__ in, T* __restrict__ out, const gplan& p, std::uint64_t tile,
+                 std::uint32_t* smem, int align)
+{
+  using uw = typename fmt<T>::uw;
+  using wt = typename word<U>::type;
+  const std::uint32_t t = threadIdx.x;
+  std::uint32_t* img = smem;
+  int* ex = reinterpret_cast<int*>(smem + image_dwords(p));
+  const std::uint64_t k0 = tile * p.TB;
+  const std::uint64_t base = k0 * p.B;
+  const std::uint64_t left = p.n - base;
+  const std::uint32_t cnt = left < p.V ? static_cast<std::uint32_t>(left) : p.V;
+  const std::uint32_t nblk = block_of(cnt - 1, p) + 1;
+  for (std::uint32_t b = t; b < nblk; b += WG) {
+    ex[b] = load_block_exponent<U>(in, k0 + b);
+  }
+  move_payload<true, NT>(const_cast<std::uint8_t*>(in) + p.R + k0 * p.CB, img, nblk * p.CB, align);
+  __syncthreads();
+  const std::uint32_t bits = static_cast<std::uint32_t>(p.bits);
+  const std::uint32_t padbits = p.CB * 8u - p.B * bits;
+  uw code[VPT];
+  int E[VPT];
+  std::uint32_t ic[VPT];
+  int emin = 0x7fffffff;
+#pragma unroll
+  for (int r = 0; r < VPT; ++r) {
+    const std::uint32_t i0 = t + static_cast<std::uint32_t>(r * WG);
+    const std::uint32_t i = i0 < cnt ? i0 : cnt - 1;
+    ic[r] = i;
+    const std::uint32_t k = block_of(i, p);
+    if constexpr (PACKED) {
+      code[r] = static_cast<uw>(tile::extract_code<U>(img, i * bits + k * padbits));
+    } else {
+      code[r] = static_cast<uw>(reinterpret_cast<const wt*>(img)[i]);
+    }
+    E[r] = ex[k];
+    emin = E[r] < emin ? E[r] : emin;
+  }
+  // decode_value_short needs bits <= SB + 1 and E + bias >= 0 (every block a
+  // compress writes); checked over all of the thread's values
+  const bool fast = p.bits <= fmt<T>::SB + 1 && emin + fmt<T>::BIAS >= 0;
+  uw* o = reinterpret_cast<uw*>(out + base);
+  if (fast) {
+#pragma unroll
+    for (int r = 0; r < VPT; ++r) {
+      const uw y = decode_value_short<T>(code[r], E[r], p.bits);
+      if (t + static_cast<std::uint32_t>(r * WG) < cnt) {
+        tile::st<NT>(y, o + ic[r]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < VPT; ++r) {
+      const uw y = decode_value<T>(code[r], E[r], p.bits);
+      if (t + static_cast<std::uint32_t>(r * WG) < cnt) {
+        tile::st<NT>(y, o + ic[r]);
+      }
+    }
+  }
+}
+
+} // namespace gtile
+} // namespace frsz2
