@@ -79,7 +79,7 @@ gather_tile(const bjob& J, std::uint64_t t)
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     if (i0 + v < J.m) {
-      out[i0 + v] = idx[v] < J.n ? decode_element<T, U>(stream, L, idx[v]) : T(0);
+      out[i0 + v] = idx[v] < J.n ? decode_element<T, U, true>(stream, L, idx[v]) : T(0);
     }
   }
 }

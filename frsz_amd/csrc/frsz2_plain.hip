@@ -241,7 +241,7 @@ __global__ __launch_bounds__(WG) void k_gather(const std::uint8_t* __restrict__ 
   const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
   if (i < m) {
     const std::uint64_t j = idx[i];
-    out[i] = j < L.n ? decode_element<T, U>(in, L, j) : T(0);
+    out[i] = j < L.n ? decode_element<T, U, true>(in, L, j) : T(0);
   }
 }
 

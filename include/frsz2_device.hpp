@@ -314,11 +314,26 @@ struct word<64>
   using type = std::uint64_t;
 };
 
+// *p, on the device with the non-temporal hint when NT (data read once at a
+// random place: keeping it out of the caches leaves them to data that is
+// read again)
+template<bool NT, typename V>
+FRSZ2_HDI V
+load_hinted(const V* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (NT) {
+    return __builtin_nontemporal_load(p);
+  }
+#endif
+  return *p;
+}
+
 // Code j of a block whose payload starts at blk: the U-bit word holding its
 // first bit, plus the next word only when the code straddles the boundary
 // (retrieve_shift_value, include/frsz2.hpp:615-627).  Never reads outside the
-// block.
-template<int U>
+// block.  NT: the word loads carry the non-temporal hint.
+template<int U, bool NT = false>
 FRSZ2_HDI std::uint64_t
 load_code(const std::uint8_t* __restrict__ blk, std::uint64_t j, int bits)
 {
@@ -327,9 +342,9 @@ load_code(const std::uint8_t* __restrict__ blk, std::uint64_t j, int bits)
   const std::uint64_t w = p / U;
   const int o = static_cast<int>(p % U);
   const wt* words = reinterpret_cast<const wt*>(blk);
-  std::uint64_t r = static_cast<std::uint64_t>(words[w]) >> o;
+  std::uint64_t r = static_cast<std::uint64_t>(load_hinted<NT>(words + w)) >> o;
   if (o + bits > U) {
-    r |= static_cast<std::uint64_t>(words[w + 1]) << (U - o);
+    r |= static_cast<std::uint64_t>(load_hinted<NT>(words + w + 1)) << (U - o);
   }
   return bits == 64 ? r : (r & ((std::uint64_t{ 1 } << bits) - 1));
 }
@@ -427,8 +442,11 @@ decode_block(const std::uint8_t* __restrict__ stream, const layout& L, std::uint
 
 // Random-access decode of element idx of a stream (decompress_cpu_element,
 // include/frsz2.hpp:1199-1214), callable from any kernel.  T = float/double,
-// U = storage_bits(L.bits).
-template<typename T, int U>
+// U = storage_bits(L.bits).  NT: the code word is loaded with the
+// non-temporal hint (for gathers over streams much larger than the caches:
+// the codes, touched once each, then do not evict the block exponents,
+// which neighbouring indices share).
+template<typename T, int U, bool NT = false>
 FRSZ2_HDI T
 decode_element(const std::uint8_t* __restrict__ stream, const layout& L, std::uint64_t idx)
 {
@@ -437,7 +455,7 @@ decode_element(const std::uint8_t* __restrict__ stream, const layout& L, std::ui
   const std::uint64_t k = (L.B & (L.B - 1)) == 0 ? idx >> __builtin_ctzll(L.B) : idx / L.B;
   const std::uint64_t j = idx - k * L.B;
   const int E = load_block_exponent<U>(stream, k);
-  const std::uint64_t code = load_code<U>(stream + L.R + k * L.CB, j, L.bits);
+  const std::uint64_t code = load_code<U, NT>(stream + L.R + k * L.CB, j, L.bits);
   const auto u = decode_value<T>(static_cast<typename fmt<T>::uw>(code), E, L.bits);
   return __builtin_bit_cast(T, u);
 }
