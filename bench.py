@@ -484,7 +484,7 @@ def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_opti
     trace, idx_all, idx_off = serving_requests(tr, src, B, keep)
     torch.cuda.synchronize()
     print(f"bench: serving {nreq} requests over {seconds:.0f} s", file=sys.stderr, flush=True)
-    opts = dict(slots=4, tile_shape=3)
+    opts = dict(slots=4)
     opts.update(server_options or {})
     srv = serve.Server(hbm_budget=budget_gib << 30, **opts)
     res, tickets = srv.replay(trace)

@@ -2,7 +2,7 @@
 
     jobs = [batch.compress_job(x, stream_buf, bits=16, block=32), ...]
     plan = batch.BatchPlan(jobs)          # descriptors uploaded, launch captured in a hipGraph
-    plan.autotune()                        # optional: time candidate launch shapes on these jobs
+    plan.autotune()                        # optional: time the tile schedules on these jobs
     plan.launch()                          # one graph launch on the current stream
 
 Each job means exactly one frsz_amd.compress_into / decompress_into call
@@ -41,14 +41,23 @@ class Job(ctypes.Structure):
     ]
 
 
+SCHEDULE_TILES = 0       # one workgroup per tile
+SCHEDULE_PERSISTENT = 1  # per class, a grid of what the device holds striding over its tiles
+
+
 class BatchConfig(ctypes.Structure):
     _fields_ = [
+        ("schedule", ctypes.c_int),
         ("workgroups", ctypes.c_int),
-        ("tile_shape", ctypes.c_int),
-        ("tiles_per_claim", ctypes.c_int),
+        ("xcd_order", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("plain_launches", ctypes.c_int),
     ]
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "BatchConfig":
+        return cls(d.get("schedule", SCHEDULE_TILES), d.get("workgroups", 0), d.get("xcd_order", 0),
+                   d.get("use_graph", 1), d.get("plain_launches", 0))
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -131,11 +140,7 @@ class BatchPlan:
         self._keep = [k for _, k in jobs]
         arr = (Job * len(jobs))(*[j for j, _ in jobs])
         handle = ctypes.c_void_p()
-        cfg = None
-        if config is not None:
-            cfg = BatchConfig(config.get("workgroups", 0), config.get("tile_shape", 3),
-                              config.get("tiles_per_claim", 4), config.get("use_graph", 1),
-                              config.get("plain_launches", 0))
+        cfg = None if config is None else BatchConfig.from_dict(config)
         check(_L.frsz2_batch_create(arr, len(jobs), ctypes.byref(cfg) if cfg is not None else None,
                                     ctypes.byref(handle)))
         self._h = handle
@@ -159,9 +164,7 @@ class BatchPlan:
     def config(self, cfg: dict) -> None:
         cur = self.config
         cur.update(cfg)
-        c = BatchConfig(cur["workgroups"], cur["tile_shape"], cur["tiles_per_claim"], cur["use_graph"],
-                        cur["plain_launches"])
-        check(_L.frsz2_batch_set_config(self._h, ctypes.byref(c)))
+        check(_L.frsz2_batch_set_config(self._h, ctypes.byref(BatchConfig.from_dict(cur))))
 
     def close(self) -> None:
         if self._h:
@@ -175,5 +178,6 @@ class BatchPlan:
             pass
 
 
-__all__ = ["BatchPlan", "run_noalloc", "NOALLOC_MAX_JOBS", "compress_job", "decompress_job", "gather_job", "compressed_size", "OP_COMPRESS",
-           "OP_DECOMPRESS", "OP_GATHER"]
+__all__ = ["BatchPlan", "BatchConfig", "run_noalloc", "NOALLOC_MAX_JOBS", "compress_job", "decompress_job",
+           "gather_job", "compressed_size", "OP_COMPRESS", "OP_DECOMPRESS", "OP_GATHER", "SCHEDULE_TILES",
+           "SCHEDULE_PERSISTENT"]

@@ -1,5 +1,5 @@
-// Internal interface of the persistent batch kernel, shared by the batch
-// plans (frsz2_batch.hip) and the serving runtime (frsz2_serve.hip).
+// Internal interface of the grouped batch kernels, shared by the batch plans
+// (frsz2_batch.hip) and the serving runtime (frsz2_serve.hip).
 #pragma once
 
 #include "frsz2_capi.h"
@@ -12,7 +12,7 @@
 
 namespace frsz2 {
 
-// device descriptor of one job of the persistent kernel (96 bytes)
+// device descriptor of one job of the grouped kernels (96 bytes)
 struct bjob
 {
   const void* in;           // values (compress) or stream (decompress, gather)
@@ -32,12 +32,12 @@ struct bjob
 };
 static_assert(sizeof(bjob) == 96, "descriptor layout");
 
-struct batch_shape
+// tile schedule of the grouped launches (frsz2_batch_config)
+struct batch_schedule
 {
-  int tile_shape;      // 0 narrow, 1 balanced, 2 wide, 3 grouped (one launch per class)
-  int workgroups;      // persistent grid
-  int tiles_per_claim; // 0 = static round-robin
-  int ops;             // directions present (1 compress, 2 decompress, 4 gather); 0 = any
+  int persistent; // 0: one workgroup per tile; 1: per class, a persistent grid striding over its tiles
+  int workgroups; // persistent: workgroups per CU (0 = as many as the device holds)
+  int xcd_order;  // 1: one contiguous range of a class's tiles per XCD; 0: workgroup order
 };
 
 // checks one job like the single-array entry points do
@@ -47,31 +47,28 @@ int validate_job(const frsz2_job& j, int idx);
 // class, job order within a class) with global tile numbers starting at 0,
 // and sets tiled[i] = 1 for the jobs they cover; returns the total tile
 // count.  The other jobs must run through run_plain.
-std::uint64_t build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vector<bjob>& desc,
+std::uint64_t build_descriptors(const frsz2_job* jobs, std::size_t n, std::vector<bjob>& desc,
                                 std::vector<char>& tiled);
 
-// Descriptors a persistent launch can carry in its kernel arguments.
+// Descriptors a launch can carry in its kernel arguments.
 constexpr int INLINE_JOBS = 8;
 
-// True when launch_persistent passes the descriptors in the kernel arguments
-// (at most INLINE_JOBS descriptors, and a grid of one workgroup per tile:
-// tile shape 3, or a persistent grid no larger than the tile count): d_desc
-// is then not read and need not hold them.
-bool descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tiles);
+// True when launch_batch passes the descriptors in the kernel arguments (at
+// most INLINE_JOBS descriptors, one workgroup per tile): d_desc is then not
+// read and need not hold them.
+bool descriptors_inline(const batch_schedule& sc, int ndesc, std::uint64_t total_tiles);
 
-// Counter reset (when claiming) + persistent kernel over ndesc descriptors in
-// device memory (d_desc; h_desc is the host copy, which is what the kernel
-// reads when descriptors_inline holds), on stream s.  Tile shape 3 launches
-// one grouped kernel per class instead.  Each tile's job is read from d_jtab
-// (tile_table of the descriptors for this tile shape, in device memory) or,
-// when that is null, found by a binary search.
-hipError_t launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
-                             unsigned long long* d_counter, const batch_shape& shape, hipStream_t s,
-                             const std::uint32_t* d_jtab = nullptr);
+// One grouped kernel per class present over the ndesc descriptors (in device
+// memory at d_desc; h_desc is the host copy, which is what the kernels read
+// when descriptors_inline holds), on stream s, with the tile schedule sc.
+// Each tile's job is read from d_jtab (tile_table of the descriptors, in
+// device memory) or, when that is null, found by a binary search.
+hipError_t launch_batch(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
+                        const batch_schedule& sc, hipStream_t s, const std::uint32_t* d_jtab = nullptr);
 
 // Entry t is the index of the descriptor that owns global tile t: within its
-// class's run of descriptors (tile shape 3, whose kernels see one class's
-// descriptors) or among all of them (the persistent shapes).
+// class's run of descriptors (within_class: the grouped kernels see one
+// class's descriptors) or among all of them.
 std::vector<std::uint32_t> tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles, bool within_class);
 
 // One job as a plain single-array launch on s.

@@ -1,21 +1,20 @@
 // Batched FRSZ2: many independent arrays of uneven length, mixed float32 /
-// float64, bit widths and directions, in one persistent launch captured in a
-// hipGraph.
+// float64, bit widths and directions, in one captured hipGraph launch.
 //
-// The jobs' tiles form one global index space (job j owns tiles
-// [tile0_j, tile0_j + ntiles_j)).  A persistent grid of workgroups claims
-// tiles from a device counter (`chunk` tiles per atomic claim; chunk 0 =
-// static round-robin, no counter), reads the tile's job from the plan's tile
-// table (or, without one, finds it by binary search over tile0), and runs
-// the same tile code as the plain kernels for the job's class (direction,
-// dtype, storage width U, packed or not).  Jobs the tile path cannot take (B
-// not a power of two or larger than a tile, unaligned pointers) run as plain
-// launches of the generic kernels inside the same graph.  The counter is
-// reset by a one-thread kernel captured at the head of the graph.
+// The jobs' tiles form one global index space, ordered by kernel class
+// (direction, dtype, storage width U, packed or not; job j owns tiles
+// [tile0_j, tile0_j + ntiles_j)).  The graph holds one grouped kernel per
+// class present, each with the plain kernels' built-in variant for that
+// class and its own register allocation, running the same tile code as the
+// plain kernels; a workgroup reads its tile's job from the plan's tile table
+// (or, without one, finds it by binary search over tile0).  Jobs the tile
+// path cannot take (unaligned pointers, blocks larger than a tile) run as
+// plain launches inside the same graph.
 //
-// Tile shapes are per class and come in three settings (narrow, balanced,
-// wide; see class_vpt).  The autotuner times candidate (workgroups, tile
-// shape, chunk) settings on the plan's own jobs and keeps the fastest.
+// The tile schedule is chosen by the autotuner on the plan's own jobs: one
+// workgroup per tile or a persistent per-class grid of what the device holds
+// at once, each in workgroup order or with one contiguous range of a
+// class's tiles per XCD.
 //
 // No counterpart in the reference: its kernels take one array per launch
 // (include/frsz2.hpp:1258-1288).
@@ -97,168 +96,19 @@ __host__ __device__ constexpr int cls_default(int cls)
   return default_variant(cls_op(cls) == 2 ? 1 : cls_op(cls), cls_dtype(cls), cls_ui(cls), cls_packed(cls));
 }
 
-// Per-class tile shape, three settings:
-//   0 narrow:   4 values per thread for every class (smallest tiles, finest
-//               load balance, fewest registers), float64 decompress staged
-//   1 balanced: up to 8 values per thread, float64 decompress through the
-//               LDS output stage
-//   2 wide:     the shapes measured fastest for the plain kernels (up to 16
-//               values per thread, staged decompress stores)
-//   3 grouped:  not one persistent kernel but one launch per class present,
-//               each with that class's default plain-kernel variant and its
-//               own register allocation (launch_grouped)
-// In one kernel every class shares the register allocation of the widest,
-// so wider shapes trade occupancy for bytes in flight; the autotuner picks.
+// Values per thread, workgroup size and staged decompress stores of a
+// class's tiles: the plain kernels' built-in variant for the class
+// (tuning_defaults.hpp); gathers take 4 indices per thread.
 __host__ __device__ constexpr int
-class_vpt(int cls, int shape)
+class_vpt(int cls)
 {
-  if (shape == 3) {
-    return venc_vpt(cls_default(cls)); // grouped: the plain kernels' default variant
-  }
-  const int op = cls / 14;
-  const int slot = cls % 14;
-  const bool f32 = slot < 6;
-  const int ui = f32 ? slot / 2 : (slot - 6) / 2;
-  const bool packed = (slot % 2) == 1;
-  if (shape == 0 || op == 2) {
-    return 4;
-  }
-  if (op == 0) {
-    return f32 ? ((ui == 2 && !packed) ? 4 : 8) : 4;
-  }
-  if (shape == 1) {
-    return f32 ? 8 : 4;
-  }
-  if (f32) {
-    return packed ? 8 : (ui == 2 ? 8 : 16);
-  }
-  return packed ? 4 : 8;
+  return cls_op(cls) == 2 ? 4 : venc_vpt(cls_default(cls));
 }
 
-// threads per workgroup of a class's tiles: 64 where the grouped shape runs
-// a plain-kernel default that uses one-wave workgroups, else 256
 __host__ __device__ constexpr int
-class_wgs(int cls, int shape)
+class_wgs(int cls)
 {
-  return shape == 3 && cls_op(cls) != 2 && venc_small(cls_default(cls)) != 0 ? 64 : WG;
-}
-
-__host__ __device__ constexpr bool
-class_stage(int cls, int shape)
-{
-  if (shape == 3) {
-    return cls_op(cls) == 1 && venc_stage(cls_default(cls)) != 0;
-  }
-  const int slot = cls % 14;
-  const bool f32 = slot < 6;
-  const bool packed = (slot % 2) == 1;
-  if (cls / 14 != 1 || cls >= 28 || packed) {
-    return false;
-  }
-  // bits == U: the strided decode loads one code per lane per step, which
-  // pays off for 4-byte codes (float32, U 32) and not for 1-2-byte ones
-  return !(f32 && slot / 2 == 2);
-}
-
-// LDS dwords of the batch kernel: the largest payload image or output stage
-// of any class (a tile uses one or the other)
-template<int SHAPE>
-constexpr int
-batch_lds_dwords()
-{
-  return SHAPE == 0 ? tile::lds_dwords<4>() : tile::lds_dwords<8>();
-}
-static_assert(tile::stage_dwords<double, 4>() <= tile::lds_dwords<4>(), "stage fits");
-static_assert(tile::stage_dwords<float, 16>() <= tile::lds_dwords<8>(), "stage fits");
-static_assert(tile::stage_dwords<double, 8>() <= tile::lds_dwords<8>(), "stage fits");
-
-// OPS: bit mask of the directions a kernel instance handles (1 compress,
-// 2 decompress, 4 gather); the code (and register allocation) of the other
-// classes is left out.
-template<int SHAPE, int OPS>
-__device__ __forceinline__ void
-run_tile(const bjob& J, std::uint64_t t, std::uint32_t* lds, std::uint32_t* wred)
-{
-  const tplan p{ J.n, J.nb, J.R, J.CB, J.bits, J.lb < 0 ? 0 : J.lb, 0 };
-  auto* o8 = static_cast<std::uint8_t*>(J.out);
-  const auto* i8 = static_cast<const std::uint8_t*>(J.in);
-#define FRSZ2_C(K, T, U, PK)                                                                                 \
-  case K:                                                                                                    \
-    if constexpr ((OPS & 1) != 0)                                                                            \
-    tile::compress_tile<T, U, class_vpt(K, SHAPE), PK, true>(static_cast<const typename fmt<T>::uw*>(J.in),   \
-                                                            o8, p, t, lds, wred);                            \
-    if constexpr ((OPS & 1) != 0)                                                                            \
-    if (t == 0)                                                                                              \
-      tile::zero_pad(o8, J.nb, U, J.R, static_cast<int>(threadIdx.x), WG);                                   \
-    break
-#define FRSZ2_D(K, T, U, PK)                                                                                 \
-  case K:                                                                                                    \
-    if constexpr ((OPS & 2) != 0)                                                                            \
-    tile::decompress_tile<T, U, class_vpt(K, SHAPE), PK, true, class_stage(K, SHAPE)>(                         \
-      i8, static_cast<typename fmt<T>::uw*>(J.out), p, t, lds, lds);                                         \
-    break
-  switch (J.cls) {
-    FRSZ2_C(0, float, 8, false);
-    FRSZ2_C(1, float, 8, true);
-    FRSZ2_C(2, float, 16, false);
-    FRSZ2_C(3, float, 16, true);
-    FRSZ2_C(4, float, 32, false);
-    FRSZ2_C(5, float, 32, true);
-    FRSZ2_C(6, double, 8, false);
-    FRSZ2_C(7, double, 8, true);
-    FRSZ2_C(8, double, 16, false);
-    FRSZ2_C(9, double, 16, true);
-    FRSZ2_C(10, double, 32, false);
-    FRSZ2_C(11, double, 32, true);
-    FRSZ2_C(12, double, 64, false);
-    FRSZ2_C(13, double, 64, true);
-    FRSZ2_D(14, float, 8, false);
-    FRSZ2_D(15, float, 8, true);
-    FRSZ2_D(16, float, 16, false);
-    FRSZ2_D(17, float, 16, true);
-    FRSZ2_D(18, float, 32, false);
-    FRSZ2_D(19, float, 32, true);
-    FRSZ2_D(20, double, 8, false);
-    FRSZ2_D(21, double, 8, true);
-    FRSZ2_D(22, double, 16, false);
-    FRSZ2_D(23, double, 16, true);
-    FRSZ2_D(24, double, 32, false);
-    FRSZ2_D(25, double, 32, true);
-    FRSZ2_D(26, double, 64, false);
-    FRSZ2_D(27, double, 64, true);
-    case 28:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<float, 8>(J, t);
-      break;
-    case 29:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<float, 16>(J, t);
-      break;
-    case 30:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<float, 32>(J, t);
-      break;
-    case 31:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<double, 8>(J, t);
-      break;
-    case 32:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<double, 16>(J, t);
-      break;
-    case 33:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<double, 32>(J, t);
-      break;
-    case 34:
-      if constexpr ((OPS & 4) != 0)
-        gather_tile<double, 64>(J, t);
-      break;
-    default:
-      break;
-  }
-#undef FRSZ2_C
-#undef FRSZ2_D
+  return cls_op(cls) != 2 && venc_small(cls_default(cls)) != 0 ? 64 : WG;
 }
 
 // job owning global tile g, searching from hint (jobs sorted by tile0)
@@ -281,89 +131,6 @@ find_job(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, int hint)
   return lo;
 }
 
-// The persistent loop: tiles claimed `chunk` at a time from *counter, or
-// (chunk == 0) taken round-robin by workgroup index.  A tile's job comes from
-// jtab (descriptor index per tile) when given, else from a search.
-template<int SHAPE, int OPS>
-__device__ __forceinline__ void
-batch_loop(const bjob* __restrict__ jobs, int njobs, std::uint64_t total_tiles, unsigned long long* counter,
-           int chunk, const std::uint32_t* __restrict__ jtab)
-{
-  __shared__ std::uint32_t lds[batch_lds_dwords<SHAPE>()];
-  __shared__ std::uint32_t wred[WG / tile::WAVE];
-  __shared__ unsigned long long claim;
-  if (chunk == 0 && jtab) {
-    // Round-robin with the job lookups run ahead: the descriptor of the next
-    // tile and the table entry of the one after it are loaded before the
-    // current tile runs, so their latency passes under its work instead of
-    // between tiles.  Every value loaded here is workgroup-uniform.
-    // (Past the last tile the lookups read the last tile's entry, so every
-    // load is unconditional and in bounds.)
-    const std::uint64_t G = gridDim.x;
-    const std::uint64_t last = total_tiles - 1;
-    std::uint64_t g = blockIdx.x;
-    if (g >= total_tiles) {
-      return;
-    }
-    bjob J = jobs[jtab[g]];
-    std::uint32_t jn = jtab[g + G < last ? g + G : last];
-    for (;;) {
-      const std::uint64_t gn = g + G;
-      const bjob Jn = jobs[jn];
-      const std::uint32_t jnn = jtab[gn + G < last ? gn + G : last];
-      run_tile<SHAPE, OPS>(J, g - J.tile0, lds, wred);
-      if (gn >= total_tiles) {
-        break;
-      }
-      g = gn;
-      J = Jn;
-      jn = jnn;
-    }
-    return;
-  }
-  int j = 0;
-  std::uint64_t next = blockIdx.x;
-  for (;;) {
-    std::uint64_t g0;
-    std::uint64_t g1;
-    if (chunk > 0) {
-      if (threadIdx.x == 0) {
-        claim = atomicAdd(counter, static_cast<unsigned long long>(chunk));
-      }
-      __syncthreads();
-      g0 = claim;
-      __syncthreads();
-      g1 = g0 + static_cast<std::uint64_t>(chunk);
-    } else {
-      g0 = next;
-      g1 = g0 + 1;
-      next += gridDim.x;
-    }
-    if (g0 >= total_tiles) {
-      break;
-    }
-    if (g1 > total_tiles) {
-      g1 = total_tiles;
-    }
-    for (std::uint64_t g = g0; g < g1; ++g) {
-      j = __builtin_amdgcn_readfirstlane(jtab ? static_cast<int>(jtab[g]) : find_job(jobs, njobs, g, j));
-      const bjob J = jobs[j];
-      run_tile<SHAPE, OPS>(J, g - J.tile0, lds, wred);
-    }
-  }
-}
-
-template<int SHAPE, int OPS>
-__global__ __launch_bounds__(WG) void k_batch(const bjob* __restrict__ jobs,
-                                              int njobs,
-                                              std::uint64_t total_tiles,
-                                              unsigned long long* counter,
-                                              int chunk,
-                                              const std::uint32_t* __restrict__ jtab)
-{
-  batch_loop<SHAPE, OPS>(jobs, njobs, total_tiles, counter, chunk, jtab);
-}
-
 // Up to INLINE_JOBS descriptors passed in the kernel arguments: a launch of
 // a few jobs needs no host-to-device copy of its descriptors before it.
 struct job_pack
@@ -371,28 +138,13 @@ struct job_pack
   bjob j[INLINE_JOBS];
 };
 
-template<int SHAPE, int OPS>
-__global__ __launch_bounds__(WG) void k_batch_inline(job_pack pack, int njobs, std::uint64_t total_tiles)
-{
-  batch_loop<SHAPE, OPS>(pack.j, njobs, total_tiles, nullptr, 0, nullptr);
-}
-
-__global__ void
-k_reset_counter(unsigned long long* counter)
-{
-  *counter = 0ull;
-}
-
-// Grouped mode: one workgroup per tile of the jobs of one class (jobs[0..njobs)
-// are that class's descriptors, tiles tile_base + blockIdx.x).
+// Grouped mode: global tile g of the jobs of one class (jobs[0..njobs) are
+// that class's descriptors), with the workgroup's LDS arrays.
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
 __device__ __forceinline__ void
-grouped_tile(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base, const std::uint32_t* __restrict__ jtab)
+grouped_tile_at(const bjob* __restrict__ jobs, int njobs, std::uint64_t g, const std::uint32_t* __restrict__ jtab,
+                std::uint32_t* lds, std::uint32_t* stage, std::uint32_t* wred)
 {
-  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
-  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
-  __shared__ std::uint32_t wred[WGS / tile::WAVE];
-  const std::uint64_t g = tile_base + blockIdx.x;
   // the plan's tile table gives the job in one load; without one, a binary
   // search over the class's descriptors
   const int j = jtab ? static_cast<int>(jtab[g]) : __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, g, 0));
@@ -417,46 +169,112 @@ grouped_tile(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base, 
   }
 }
 
+// the class's tile of virtual index v < ntiles: v itself (workgroup order)
+// or, with xcd, the XCD-contiguous order (workgroups are dealt round-robin to
+// the 8 XCDs; each XCD then takes one contiguous range of the class's tiles)
+__device__ __forceinline__ std::uint64_t
+class_tile(std::uint64_t v, std::uint64_t ntiles, int xcd)
+{
+  return xcd ? tile::xcd_tile(static_cast<std::uint32_t>(v), static_cast<std::uint32_t>(ntiles)) : v;
+}
+
+// one workgroup per tile: virtual tile blockIdx.x of the class's ntiles
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
+__device__ __forceinline__ void
+grouped_tile(const bjob* __restrict__ jobs, int njobs, std::uint64_t tile_base, std::uint64_t ntiles, int xcd,
+             const std::uint32_t* __restrict__ jtab)
+{
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
+  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
+  __shared__ std::uint32_t wred[WGS / tile::WAVE];
+  grouped_tile_at<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(jobs, njobs, tile_base + class_tile(blockIdx.x, ntiles, xcd),
+                                                         jtab, lds, stage, wred);
+}
+
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
 __global__ __launch_bounds__(WGS) void k_grouped(const bjob* __restrict__ jobs,
                                                  int njobs,
                                                  std::uint64_t tile_base,
+                                                 std::uint64_t ntiles,
+                                                 int xcd,
                                                  const std::uint32_t* __restrict__ jtab)
 {
-  grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(jobs, njobs, tile_base, jtab);
+  grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(jobs, njobs, tile_base, ntiles, xcd, jtab);
+}
+
+// Persistent: a grid of at most what the device holds at once strides over
+// the class's ntiles tiles: workgroup w takes virtual tiles w, w + grid, ...
+// (with xcd the grid is a multiple of 8, so a workgroup's virtual tiles all
+// map into its own XCD's range).  Every tile function ends with the barrier
+// that frees its LDS for the next tile.
+template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
+__global__ __launch_bounds__(WGS) void k_grouped_persistent(const bjob* __restrict__ jobs,
+                                                            int njobs,
+                                                            std::uint64_t tile_base,
+                                                            std::uint64_t ntiles,
+                                                            int xcd,
+                                                            const std::uint32_t* __restrict__ jtab)
+{
+  __shared__ std::uint32_t lds[PACKED ? tile::lds_dwords_u<U, VPT, WGS>() : 1];
+  __shared__ std::uint32_t stage[STAGE ? tile::stage_dwords<T, VPT, WGS>() : 1];
+  __shared__ std::uint32_t wred[WGS / tile::WAVE];
+  for (std::uint64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    grouped_tile_at<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(jobs, njobs, tile_base + class_tile(v, ntiles, xcd), jtab,
+                                                           lds, stage, wred);
+  }
 }
 
 // the class's descriptors (at most INLINE_JOBS) in the kernel arguments
 template<typename T, int U, int VPT, bool PACKED, bool NT, bool STAGE, int OP, int WGS>
-__global__ __launch_bounds__(WGS) void k_grouped_inline(job_pack pack, int njobs, std::uint64_t tile_base)
+__global__ __launch_bounds__(WGS) void k_grouped_inline(job_pack pack,
+                                                        int njobs,
+                                                        std::uint64_t tile_base,
+                                                        std::uint64_t ntiles,
+                                                        int xcd)
 {
-  grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(pack.j, njobs, tile_base, nullptr);
+  grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(pack.j, njobs, tile_base, ntiles, xcd, nullptr);
 }
 
-// h: host copy of the class's descriptors when they go in the kernel
-// arguments (nd <= INLINE_JOBS), else null
+// The grouped kernel of class CLS over its descriptors d[0..nd), tiles
+// [t0, t0 + nt).  h: host copy of the descriptors when they go in the kernel
+// arguments (nd <= INLINE_JOBS, one workgroup per tile), else null.
 template<int CLS>
 void
 launch_class(const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64_t nt, const std::uint32_t* jtab,
-             hipStream_t s)
+             hipStream_t s, const batch_schedule& sc)
 {
   using T = std::conditional_t<cls_dtype(CLS) == 0, float, double>;
   constexpr int U = 8 << cls_ui(CLS);
   constexpr int V = cls_default(CLS);
   constexpr int OP = cls_op(CLS);
-  constexpr int VPT = OP == 2 ? 4 : venc_vpt(V);
+  constexpr int VPT = class_vpt(CLS);
   constexpr bool PACKED = cls_packed(CLS) != 0;
   constexpr bool NT = venc_nt(V) != 0;
   constexpr bool STAGE = OP == 1 && venc_stage(V) != 0;
-  constexpr int WGS = class_wgs(CLS, 3);
-  if (h != nullptr) {
+  constexpr int WGS = class_wgs(CLS);
+  const int xcd = sc.xcd_order ? 1 : 0;
+  if (sc.persistent) {
+    const auto kern = k_grouped_persistent<T, U, VPT, PACKED, NT, STAGE, OP, WGS>;
+    // workgroups of this kernel resident per CU (registers, LDS, waves), once per process
+    static const int resident = [kern]() {
+      int b = 0;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, WGS, 0) == hipSuccess && b > 0 ? b : 4;
+    }();
+    const int per_cu = sc.workgroups > 0 && sc.workgroups < resident ? sc.workgroups : resident;
+    const std::uint64_t cap = static_cast<std::uint64_t>(per_cu) * static_cast<std::uint64_t>(device_cus());
+    std::uint64_t grid = nt < cap ? nt : cap;
+    if (xcd && grid >= 8) {
+      grid = grid / 8 * 8;
+    }
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(WGS), 0, s, d, nd, t0, nt, xcd, jtab);
+  } else if (h != nullptr) {
     job_pack pack{};
     std::copy(h, h + nd, pack.j);
     hipLaunchKernelGGL((k_grouped_inline<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)),
-                       dim3(WGS), 0, s, pack, nd, t0);
+                       dim3(WGS), 0, s, pack, nd, t0, nt, xcd);
   } else {
     hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)),
-                       dim3(WGS), 0, s, d, nd, t0, jtab);
+                       dim3(WGS), 0, s, d, nd, t0, nt, xcd, jtab);
   }
 }
 
@@ -479,17 +297,15 @@ aligned16(const void* p)
   return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0;
 }
 
-// grouped mode: one k_grouped launch per run of same-class descriptors
-// (descriptors are built in class order)
 // the grouped kernel of class cls over descriptors d[0..nd) (h: their host
 // copy when they go in the kernel arguments), tiles [t0, t0 + nt)
 hipError_t
 launch_class_of(int cls, const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64_t nt,
-                const std::uint32_t* jtab, hipStream_t s)
+                const std::uint32_t* jtab, hipStream_t s, const batch_schedule& sc)
 {
 #define FRSZ2_G(K)                                                                                           \
   case K:                                                                                                    \
-    launch_class<K>(d, h, nd, t0, nt, jtab, s);                                                              \
+    launch_class<K>(d, h, nd, t0, nt, jtab, s, sc);                                                          \
     break
   switch (cls) {
     FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
@@ -504,9 +320,11 @@ launch_class_of(int cls, const bjob* d, const bjob* h, int nd, std::uint64_t t0,
   return hipGetLastError();
 }
 
+// one grouped launch per run of same-class descriptors (descriptors are
+// built in class order)
 hipError_t
 launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, const std::uint32_t* jtab,
-               bool inl, hipStream_t s)
+               bool inl, hipStream_t s, const batch_schedule& sc)
 {
   int d0 = 0;
   while (d0 < nd) {
@@ -519,7 +337,7 @@ launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t tot
     const std::uint64_t t1 = d1 < nd ? h_desc[d1].tile0 : total_tiles;
     if (t1 > t0) {
       const hipError_t e =
-        launch_class_of(cls, d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s);
+        launch_class_of(cls, d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s, sc);
       if (e != hipSuccess) {
         return e;
       }
@@ -571,12 +389,12 @@ validate_job(const frsz2_job& j, int idx)
   return FRSZ2_OK;
 }
 
-// Descriptor of job j for the tile path at tile_shape, its tiles starting at
-// global tile tile0: the job's tile count, or -1 when the tile path cannot
-// take it (B not a power of two or larger than the class's tile, pointers
-// off 16-byte alignment), in which case it runs as a plain launch.
+// Descriptor of job j for the tile path, its tiles starting at global tile
+// tile0: the job's tile count, or -1 when the tile path cannot take it (B
+// not a power of two or larger than the class's tile, pointers off 16-byte
+// alignment), in which case it runs as a plain launch.
 std::int64_t
-describe_job(const frsz2_job& j, int tile_shape, std::uint64_t tile0, bjob* d)
+describe_job(const frsz2_job& j, std::uint64_t tile0, bjob* d)
 {
   const int cls = class_of(j.op, j.dtype, j.bits);
   const layout L = layout::make(j.bits, j.block, j.n);
@@ -585,7 +403,7 @@ describe_job(const frsz2_job& j, int tile_shape, std::uint64_t tile0, bjob* d)
   if (j.op == FRSZ2_OP_GATHER) {
     nt = (j.m + GATHER_TILE - 1) / GATHER_TILE;
   } else {
-    const std::uint64_t TILE = static_cast<std::uint64_t>(class_wgs(cls, tile_shape)) * class_vpt(cls, tile_shape);
+    const std::uint64_t TILE = static_cast<std::uint64_t>(class_wgs(cls)) * class_vpt(cls);
     if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
       return -1;
     }
@@ -612,8 +430,7 @@ describe_job(const frsz2_job& j, int tile_shape, std::uint64_t tile0, bjob* d)
 }
 
 std::uint64_t
-build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vector<bjob>& desc,
-                  std::vector<char>& tiled)
+build_descriptors(const frsz2_job* jobs, std::size_t n, std::vector<bjob>& desc, std::vector<char>& tiled)
 {
   tiled.assign(n, 0);
   std::uint64_t tiles = 0;
@@ -628,7 +445,7 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, int tile_shape, std::vec
   });
   for (const std::size_t i : order) {
     bjob d;
-    const std::int64_t nt = describe_job(jobs[i], tile_shape, tiles, &d);
+    const std::int64_t nt = describe_job(jobs[i], tiles, &d);
     if (nt < 0) {
       continue;
     }
@@ -660,68 +477,22 @@ tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles, bool within
   return tab;
 }
 
-static std::uint64_t
-persistent_grid(const batch_shape& shape, std::uint64_t total_tiles)
-{
-  const std::uint64_t grid = static_cast<std::uint64_t>(shape.workgroups);
-  return grid < total_tiles ? grid : total_tiles;
-}
-
 bool
-descriptors_inline(const batch_shape& shape, int ndesc, std::uint64_t total_tiles)
+descriptors_inline(const batch_schedule& sc, int ndesc, std::uint64_t total_tiles)
 {
-  if (ndesc <= 0 || ndesc > INLINE_JOBS || total_tiles == 0) {
-    return false;
-  }
-  return shape.tile_shape == 3 || persistent_grid(shape, total_tiles) == total_tiles;
+  // the persistent kernels read device descriptors
+  return ndesc > 0 && ndesc <= INLINE_JOBS && total_tiles > 0 && !sc.persistent;
 }
 
 hipError_t
-launch_persistent(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles,
-                  unsigned long long* d_counter, const batch_shape& shape, hipStream_t s, const std::uint32_t* d_jtab)
+launch_batch(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles, const batch_schedule& sc,
+             hipStream_t s, const std::uint32_t* d_jtab)
 {
   if (ndesc == 0 || total_tiles == 0) {
     return hipSuccess;
   }
-  if (shape.tile_shape == 3) {
-    return launch_grouped(d_desc, h_desc, ndesc, total_tiles, d_jtab,
-                          h_desc != nullptr && descriptors_inline(shape, ndesc, total_tiles), s);
-  }
-  const std::uint64_t grid = persistent_grid(shape, total_tiles);
-  // a grid that covers every tile takes one tile per workgroup: no claims,
-  // so no counter to reset
-  const int chunk = grid == total_tiles ? 0 : shape.tiles_per_claim;
-  const bool inl = h_desc != nullptr && descriptors_inline(shape, ndesc, total_tiles);
-  if (chunk > 0) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
-  }
-  const dim3 gd(static_cast<unsigned>(grid));
-  job_pack pack{};
-  if (inl) {
-    std::copy(h_desc, h_desc + ndesc, pack.j);
-  }
-  // one kernel instance per (tile shape, set of directions present)
-  const int ops = shape.ops == 0 ? 7 : shape.ops == 1 ? 1 : (shape.ops & 1) != 0 ? 7 : 6;
-#define FRSZ2_KB(S, O)                                                                                       \
-  if (inl) {                                                                                                 \
-    hipLaunchKernelGGL((k_batch_inline<S, O>), gd, dim3(WG), 0, s, pack, ndesc, total_tiles);                \
-  } else {                                                                                                   \
-    hipLaunchKernelGGL((k_batch<S, O>), gd, dim3(WG), 0, s, d_desc, ndesc, total_tiles, d_counter, chunk,    \
-                       d_jtab);                                                                              \
-  }
-  switch (shape.tile_shape * 8 + ops) {
-    case 0 * 8 + 1: FRSZ2_KB(0, 1); break;
-    case 0 * 8 + 6: FRSZ2_KB(0, 6); break;
-    case 0 * 8 + 7: FRSZ2_KB(0, 7); break;
-    case 1 * 8 + 1: FRSZ2_KB(1, 1); break;
-    case 1 * 8 + 6: FRSZ2_KB(1, 6); break;
-    case 1 * 8 + 7: FRSZ2_KB(1, 7); break;
-    case 2 * 8 + 1: FRSZ2_KB(2, 1); break;
-    case 2 * 8 + 6: FRSZ2_KB(2, 6); break;
-    default: FRSZ2_KB(2, 7); break;
-  }
-#undef FRSZ2_KB
-  return hipGetLastError();
+  return launch_grouped(d_desc, h_desc, ndesc, total_tiles, d_jtab,
+                        h_desc != nullptr && descriptors_inline(sc, ndesc, total_tiles), s, sc);
 }
 
 int
@@ -754,37 +525,26 @@ device_cus()
 struct frsz2_batch_plan
 {
   std::vector<frsz2_job> jobs;        // as given
-  std::vector<char> tiled;            // job index -> 1 if it takes the persistent kernel
-  std::vector<frsz2::bjob> host_desc; // descriptors for cfg.tile_shape
+  std::vector<char> tiled;            // job index -> 1 if it takes the grouped tile kernels
+  std::vector<frsz2::bjob> host_desc; // descriptors, in class order
   frsz2::bjob* d_desc = nullptr;
-  std::uint32_t* d_jtab = nullptr; // tile -> descriptor index (within its class for tile shape 3)
-  unsigned long long* d_counter = nullptr;
+  std::uint32_t* d_jtab = nullptr; // tile -> descriptor index within its class
   int ndesc = 0;
   std::uint64_t total_tiles = 0;
   frsz2_batch_config cfg{};
-  int cus = 256;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int ops = 0; // directions present among the jobs (1 compress, 2 decompress, 4 gather)
 };
 
 namespace frsz2 {
 namespace {
 
-// (re)build and upload the descriptors for cfg.tile_shape
+// build and upload the descriptors and the tile table (once per plan)
 int
 upload_descriptors(frsz2_batch_plan* P)
 {
   P->host_desc.clear();
-  P->total_tiles = build_descriptors(P->jobs.data(), P->jobs.size(), P->cfg.tile_shape, P->host_desc, P->tiled);
-  if (P->d_desc) {
-    (void)hipFree(P->d_desc);
-    P->d_desc = nullptr;
-  }
-  if (P->d_jtab) {
-    (void)hipFree(P->d_jtab);
-    P->d_jtab = nullptr;
-  }
+  P->total_tiles = build_descriptors(P->jobs.data(), P->jobs.size(), P->host_desc, P->tiled);
   P->ndesc = static_cast<int>(P->host_desc.size());
   if (P->ndesc > 0) {
     hipError_t e = hipMalloc(&P->d_desc, P->host_desc.size() * sizeof(bjob));
@@ -798,7 +558,7 @@ upload_descriptors(frsz2_batch_plan* P)
     if (P->total_tiles > 0) {
       // the tile's job by table (4 bytes per tile of at least 1 KiB of data)
       // instead of a search over thousands of jobs
-      const std::vector<std::uint32_t> tab = tile_table(P->host_desc, P->total_tiles, P->cfg.tile_shape == 3);
+      const std::vector<std::uint32_t> tab = tile_table(P->host_desc, P->total_tiles, true);
       e = hipMalloc(&P->d_jtab, tab.size() * sizeof(std::uint32_t));
       if (e != hipSuccess) {
         return hip_status(e, "hipMalloc");
@@ -825,9 +585,8 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
     }
     return FRSZ2_OK;
   }
-  const batch_shape shape{ P->cfg.tile_shape, P->cfg.workgroups, P->cfg.tiles_per_claim, P->ops };
-  const hipError_t e =
-    launch_persistent(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, P->d_counter, shape, s, P->d_jtab);
+  const batch_schedule sc{ P->cfg.schedule, P->cfg.workgroups, P->cfg.xcd_order };
+  const hipError_t e = launch_batch(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, sc, s, P->d_jtab);
   if (e != hipSuccess) {
     return hip_status(e, "batch launch");
   }
@@ -891,24 +650,14 @@ capture_graph(frsz2_batch_plan* P)
 int
 apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
 {
-  frsz2_batch_config n = c;
-  if (n.tile_shape < 0 || n.tile_shape > 3) {
-    n.tile_shape = 3;
+  P->cfg = c;
+  if (P->cfg.schedule != FRSZ2_SCHEDULE_PERSISTENT) {
+    P->cfg.schedule = FRSZ2_SCHEDULE_TILES;
   }
-  if (n.workgroups <= 0) {
-    n.workgroups = 4 * P->cus;
+  if (P->cfg.workgroups < 0) {
+    P->cfg.workgroups = 0;
   }
-  if (n.tiles_per_claim < 0) {
-    n.tiles_per_claim = 0;
-  }
-  const bool rebuild = n.tile_shape != P->cfg.tile_shape || P->host_desc.empty();
-  P->cfg = n;
-  if (rebuild) {
-    const int rc = upload_descriptors(P);
-    if (rc) {
-      return rc;
-    }
-  }
+  P->cfg.xcd_order = P->cfg.xcd_order ? 1 : 0;
   return capture_graph(P);
 }
 
@@ -921,9 +670,6 @@ destroy(frsz2_batch_plan* P)
   }
   if (P->d_jtab) {
     (void)hipFree(P->d_jtab);
-  }
-  if (P->d_counter) {
-    (void)hipFree(P->d_counter);
   }
   delete P;
 }
@@ -950,27 +696,22 @@ frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config
   }
   auto* P = new frsz2_batch_plan();
   P->jobs.assign(jobs, jobs + njobs);
-  P->cus = device_cus();
-  for (size_t i = 0; i < njobs; ++i) {
-    P->ops |= 1 << jobs[i].op;
-  }
-  hipError_t e = hipMalloc(&P->d_counter, sizeof(unsigned long long));
-  if (e != hipSuccess) {
+  int rc = upload_descriptors(P);
+  if (rc) {
     destroy(P);
-    return hip_status(e, "hipMalloc");
+    return rc;
   }
   frsz2_batch_config c{};
   if (cfg) {
     c = *cfg;
   } else {
+    c.schedule = FRSZ2_SCHEDULE_TILES;
     c.workgroups = 0;
-    c.tile_shape = 3;
-    c.tiles_per_claim = 4;
+    c.xcd_order = 0;
     c.use_graph = 1;
     c.plain_launches = 0;
   }
-  P->cfg.tile_shape = -1; // force a descriptor build
-  const int rc = apply_config(P, c);
+  rc = apply_config(P, c);
   if (rc) {
     destroy(P);
     return rc;
@@ -1033,47 +774,41 @@ frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, fr
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     return set_error(FRSZ2_EHIP, "hipEventCreate failed");
   }
+  // the tile schedules: one workgroup per tile or a persistent grid, each in
+  // workgroup order or XCD order
   frsz2_batch_config best = plan->cfg;
   float best_ms = -1.0f;
-  const int shapes[4] = { 0, 1, 2, 3 };
-  const int wg_per_cu[4] = { 2, 4, 8, 16 };
-  const int chunks[4] = { 0, 1, 4, 16 };
   int rc = FRSZ2_OK;
-  for (int vi = 0; vi < 4 && rc == FRSZ2_OK; ++vi) {
-    for (int wi = 0; wi < 4 && rc == FRSZ2_OK; ++wi) {
-      for (int ci = 0; ci < 4 && rc == FRSZ2_OK; ++ci) {
-        if (shapes[vi] == 3 && (wi > 0 || ci > 0)) {
-          continue; // grouped launches have no grid or claim setting
-        }
-        frsz2_batch_config c{};
-        c.tile_shape = shapes[vi];
-        c.workgroups = wg_per_cu[wi] * plan->cus;
-        c.tiles_per_claim = chunks[ci];
-        c.use_graph = 0;
-        c.plain_launches = 0;
-        rc = apply_config(plan, c);
-        if (rc) {
-          break;
-        }
-        rc = enqueue(plan, s); // warm-up
-        if (rc) {
-          break;
-        }
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps && rc == FRSZ2_OK; ++r) {
-          rc = enqueue(plan, s);
-        }
-        (void)hipEventRecord(e1, s);
-        if (hipEventSynchronize(e1) != hipSuccess) {
-          rc = set_error(FRSZ2_EHIP, "hipEventSynchronize failed");
-          break;
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (best_ms < 0.0f || ms < best_ms) {
-          best_ms = ms;
-          best = c;
-        }
+  for (int sched = 0; sched < 2 && rc == FRSZ2_OK; ++sched) {
+    for (int xcd = 0; xcd < 2 && rc == FRSZ2_OK; ++xcd) {
+      frsz2_batch_config c{};
+      c.schedule = sched;
+      c.workgroups = 0;
+      c.xcd_order = xcd;
+      c.use_graph = 0;
+      c.plain_launches = 0;
+      rc = apply_config(plan, c);
+      if (rc) {
+        break;
+      }
+      rc = enqueue(plan, s); // warm-up
+      if (rc) {
+        break;
+      }
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps && rc == FRSZ2_OK; ++r) {
+        rc = enqueue(plan, s);
+      }
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = set_error(FRSZ2_EHIP, "hipEventSynchronize failed");
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (best_ms < 0.0f || ms < best_ms) {
+        best_ms = ms;
+        best = c;
       }
     }
   }
@@ -1104,9 +839,9 @@ frsz2_batch_destroy(frsz2_batch_plan* plan)
 
 // ---------------------------------------------------------------- allocation-free batched launch
 //
-// frsz2_batch_run_noalloc runs a batch the way a plan with the grouped shape
-// does (one launch per kernel class present, one workgroup per tile, the
-// plain kernels' variants), without allocating: the descriptors of the
+// frsz2_batch_run_noalloc runs a batch the way a plan with the default
+// schedule does (one launch per kernel class present, one workgroup per tile,
+// the plain kernels' variants), without allocating: the descriptors of the
 // tiled jobs live in `scratch`, a file-scope device array of
 // NOALLOC_SCRATCH_BYTES that every call on a device shares; a workgroup
 // finds its tile's job by binary search over its class's descriptors (no
@@ -1199,7 +934,7 @@ frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t stre
   std::size_t ndesc = 0;
   for (size_t i = 0; i < njobs; ++i) {
     bjob d;
-    if (describe_job(jobs[i], 3, 0, &d) > 0) {
+    if (describe_job(jobs[i], 0, &d) > 0) {
       ++ndesc;
     }
   }
@@ -1232,7 +967,7 @@ frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t stre
         continue;
       }
       bjob d;
-      const std::int64_t nt = describe_job(j, 3, tiles, &d);
+      const std::int64_t nt = describe_job(j, tiles, &d);
       if (nt <= 0) {
         continue;
       }
@@ -1251,7 +986,8 @@ frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t stre
     if (pos > d0) {
       hipError_t e = hipGetLastError();
       if (e == hipSuccess) {
-        e = launch_class_of(cls, d_scratch + d0, nullptr, static_cast<int>(pos - d0), t0, tiles - t0, nullptr, s);
+        e = launch_class_of(cls, d_scratch + d0, nullptr, static_cast<int>(pos - d0), t0, tiles - t0, nullptr, s,
+                            batch_schedule{ FRSZ2_SCHEDULE_TILES, 0, 0 });
       }
       if (e != hipSuccess) {
         return hip_status(e, "allocation-free batch launch");
@@ -1261,7 +997,7 @@ frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t stre
   // jobs outside the tile path: plain launches on the same stream
   for (size_t i = 0; i < njobs; ++i) {
     bjob d;
-    if (describe_job(jobs[i], 3, 0, &d) < 0) {
+    if (describe_job(jobs[i], 0, &d) < 0) {
       const int rc = run_plain(jobs[i], s);
       if (rc) {
         return rc;

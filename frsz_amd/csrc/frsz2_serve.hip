@@ -13,8 +13,10 @@
 //   fit stops admission until memory frees (so large requests are not
 //   starved).
 // * Continuous batching: whenever one of `slots` HIP streams is free, every
-//   admissible request goes into one launch of the persistent batch kernel
-//   on that stream (descriptors copied from pinned host memory).  Up to
+//   admissible request goes into one batch launch on that stream (the
+//   grouped kernels of frsz2_batch_launch: one per class present; up to 8
+//   descriptors travel in the kernel arguments, more are copied from pinned
+//   host memory).  Up to
 //   `slots` launches run on the device at once; requests that arrive while
 //   they run are admitted into the next launch as slots and memory free.
 // * Completion: the scheduler polls each slot's event; requests complete,
@@ -149,7 +151,6 @@ struct slot
   hipEvent_t done = nullptr;
   bjob* h_desc = nullptr; // pinned
   bjob* d_desc = nullptr;
-  unsigned long long* d_counter = nullptr;
   std::vector<std::uint64_t> tickets;
   std::vector<frsz2_job> jobs;
   std::vector<bjob> desc;
@@ -359,23 +360,19 @@ int
 launch_slot(frsz2_server* S, slot& sl)
 {
   sl.desc.clear();
-  const std::uint64_t tiles = build_descriptors(sl.jobs.data(), sl.jobs.size(), S->cfg.tile_shape, sl.desc, sl.tiled);
+  const std::uint64_t tiles = build_descriptors(sl.jobs.data(), sl.jobs.size(), sl.desc, sl.tiled);
   const int nd = static_cast<int>(sl.desc.size());
   if (nd > 0) {
-    int ops = 0;
-    for (const frsz2_job& j : sl.jobs) {
-      ops |= 1 << j.op;
-    }
-    const batch_shape shape{ S->cfg.tile_shape, S->cfg.workgroups, S->cfg.tiles_per_claim, ops };
+    const batch_schedule sc{ S->cfg.schedule, S->cfg.workgroups, S->cfg.xcd_order };
     hipError_t e = hipSuccess;
-    if (!descriptors_inline(shape, nd, tiles)) {
+    if (!descriptors_inline(sc, nd, tiles)) {
       std::copy(sl.desc.begin(), sl.desc.end(), sl.h_desc);
       e = hipMemcpyAsync(sl.d_desc, sl.h_desc, nd * sizeof(bjob), hipMemcpyHostToDevice, sl.stream);
       if (e != hipSuccess) {
         return hip_status(e, "hipMemcpyAsync");
       }
     }
-    e = launch_persistent(sl.d_desc, sl.desc.data(), nd, tiles, sl.d_counter, shape, sl.stream);
+    e = launch_batch(sl.d_desc, sl.desc.data(), nd, tiles, sc, sl.stream);
     if (e != hipSuccess) {
       return hip_status(e, "batch launch");
     }
@@ -566,9 +563,6 @@ destroy_server(frsz2_server* S)
     if (sl.d_desc) {
       (void)hipFree(sl.d_desc);
     }
-    if (sl.d_counter) {
-      (void)hipFree(sl.d_counter);
-    }
   }
   if (S->pool) {
     (void)hipFree(S->pool);
@@ -603,15 +597,13 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
   if (S->cfg.max_batch <= 0) {
     S->cfg.max_batch = 1024;
   }
-  if (S->cfg.tile_shape < 0 || S->cfg.tile_shape > 3) {
-    S->cfg.tile_shape = 3;
+  if (S->cfg.schedule != FRSZ2_SCHEDULE_PERSISTENT) {
+    S->cfg.schedule = FRSZ2_SCHEDULE_TILES;
   }
-  if (S->cfg.workgroups <= 0) {
-    S->cfg.workgroups = 4 * device_cus();
+  if (S->cfg.workgroups < 0) {
+    S->cfg.workgroups = 0;
   }
-  if (S->cfg.tiles_per_claim < 0) {
-    S->cfg.tiles_per_claim = 0;
-  }
+  S->cfg.xcd_order = S->cfg.xcd_order ? 1 : 0;
   if (S->cfg.stall_ms == 0) {
     S->cfg.stall_ms = 10000;
   }
@@ -628,8 +620,7 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
         (e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_desc), S->cfg.max_batch * sizeof(bjob),
                            hipHostMallocDefault)) != hipSuccess ||
-        (e = hipMalloc(&sl.d_desc, S->cfg.max_batch * sizeof(bjob))) != hipSuccess ||
-        (e = hipMalloc(&sl.d_counter, sizeof(unsigned long long))) != hipSuccess) {
+        (e = hipMalloc(&sl.d_desc, S->cfg.max_batch * sizeof(bjob))) != hipSuccess) {
       destroy_server(S);
       return hip_status(e, "server setup");
     }

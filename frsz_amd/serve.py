@@ -29,7 +29,7 @@ REQ_DISCARD = 1
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("slots", ctypes.c_int), ("max_batch", ctypes.c_int),
-                ("tile_shape", ctypes.c_int), ("workgroups", ctypes.c_int), ("tiles_per_claim", ctypes.c_int),
+                ("schedule", ctypes.c_int), ("workgroups", ctypes.c_int), ("xcd_order", ctypes.c_int),
                 ("stall_ms", ctypes.c_int)]
 
 
@@ -89,9 +89,9 @@ def result_dict(r: Result) -> dict:
 
 
 class Server:
-    def __init__(self, hbm_budget: int = 64 << 30, slots: int = 4, max_batch: int = 1024, tile_shape: int = 3,
-                 workgroups: int = 0, tiles_per_claim: int = 4):
-        cfg = ServerConfig(hbm_budget, slots, max_batch, tile_shape, workgroups, tiles_per_claim, 0)
+    def __init__(self, hbm_budget: int = 64 << 30, slots: int = 4, max_batch: int = 1024, schedule: int = 0,
+                 workgroups: int = 0, xcd_order: int = 0):
+        cfg = ServerConfig(hbm_budget, slots, max_batch, schedule, workgroups, xcd_order, 0)
         h = ctypes.c_void_p()
         check(_L.frsz2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -211,11 +211,14 @@ int frsz2_tune_auto(int op,
                     int* flags,
                     frsz2_stream_t stream);
 
-/* ---- batched: many arrays in one persistent, graph-captured launch ----
- * No counterpart in the reference: its kernels take one array per launch
- * (include/frsz2.hpp:1258-1288).  Each job has the meaning of one
- * frsz2_compress / frsz2_decompress / frsz2_gather call (same checks, same
- * bytes). */
+/* ---- batched: many arrays in one graph-captured launch ----
+ * The graph holds one grouped kernel per class of jobs present (direction,
+ * dtype, storage width, packed or not), each with the plain kernels' variant
+ * for that class; the tile schedule is picked by frsz2_batch_autotune on the
+ * plan's own jobs.  No counterpart in the reference: its kernels take one
+ * array per launch (include/frsz2.hpp:1258-1288).  Each job has the meaning
+ * of one frsz2_compress / frsz2_decompress / frsz2_gather call (same checks,
+ * same bytes). */
 
 enum
 {
@@ -241,41 +244,43 @@ typedef struct frsz2_job
   uint64_t m;           /* gather: index count */
 } frsz2_job;
 
+/* Tile schedules of the batched launch (frsz2_batch_config.schedule) */
+enum
+{
+  FRSZ2_SCHEDULE_TILES = 0,     /* one workgroup per tile (the default) */
+  FRSZ2_SCHEDULE_PERSISTENT = 1 /* per class, a grid of what the device holds at
+                                   once striding over the class's tiles */
+};
+
 typedef struct frsz2_batch_config
 {
-  int workgroups;        /* persistent grid size (0 = 4 per CU) */
-  int tile_shape;        /* 0 narrow: 4 values per thread for every class;
-                            1 balanced: up to 8, float64 decompress staged;
-                            2 wide: the plain kernels' fastest shapes (up to
-                            16 values per thread, staged decompress stores);
-                            3 grouped: instead of one persistent kernel, one
-                            launch per kernel class present (same graph), each
-                            with that class's plain-kernel default variant
-                            and one workgroup per tile (the default: the
-                            persistent shapes keep one tile in flight per
-                            workgroup and stream at 3-5 TB/s, the grouped
-                            launches at the plain kernels' rate); out of
-                            range = 3 */
-  int tiles_per_claim;   /* tiles per atomic claim; 0 = static round-robin */
-  int use_graph;         /* 1 = launch through a captured hipGraph */
-  int plain_launches;    /* 1 = no persistent kernel: one plain frsz2_compress /
-                            frsz2_decompress launch per job, in job order (the
-                            per-array baseline the batch is measured against) */
+  int schedule;        /* FRSZ2_SCHEDULE_TILES or FRSZ2_SCHEDULE_PERSISTENT
+                          (other values = FRSZ2_SCHEDULE_TILES) */
+  int workgroups;      /* persistent schedule: workgroups per CU (0, or more
+                          than fit = as many as the device holds) */
+  int xcd_order;       /* 1: each of the 8 XCDs takes one contiguous range of
+                          a class's tiles; 0: workgroup order */
+  int use_graph;       /* 1 = launch through a captured hipGraph */
+  int plain_launches;  /* 1 = no grouped kernels: one plain frsz2_compress /
+                          frsz2_decompress / frsz2_gather launch per job, in
+                          job order (the per-array baseline the batch is
+                          measured against) */
 } frsz2_batch_config;
 
 typedef struct frsz2_batch_plan frsz2_batch_plan;
 
-/* Validate the jobs, upload their descriptors and capture the launch.
- * cfg NULL = defaults (grouped launches, graph on; for the persistent
- * shapes 4 workgroups per CU and 4 tiles per claim).  Jobs must not overlap in memory. */
+/* Validate the jobs, upload their descriptors and tile table, and capture
+ * the launch.  cfg NULL = defaults (one workgroup per tile, workgroup order,
+ * graph on).  Jobs must not overlap in memory. */
 int frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config* cfg,
                        frsz2_batch_plan** plan);
-/* Run every job of the plan, stream-ordered.  Launches of one plan must not
- * overlap in time (they share the plan's tile counter). */
+/* Run every job of the plan, stream-ordered.  Launches of one plan that
+ * overlap in time write the same outputs concurrently, so order them. */
 int frsz2_batch_launch(frsz2_batch_plan* plan, frsz2_stream_t stream);
-/* Time candidate configurations on the plan's jobs (each launched reps
- * times on stream; outputs are overwritten with identical bytes), keep the
- * fastest and re-capture; the choice is written to *chosen if not NULL. */
+/* Time the tile schedules on the plan's jobs (one workgroup per tile and
+ * persistent, each in workgroup and XCD order; each launched reps times on
+ * stream, outputs overwritten with identical bytes), keep the fastest and
+ * re-capture; the choice is written to *chosen if not NULL. */
 int frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, frsz2_batch_config* chosen);
 int frsz2_batch_get_config(const frsz2_batch_plan* plan, frsz2_batch_config* cfg);
 int frsz2_batch_set_config(frsz2_batch_plan* plan, const frsz2_batch_config* cfg);
@@ -302,7 +307,7 @@ int frsz2_batch_run_noalloc(const frsz2_job* jobs, size_t njobs, frsz2_stream_t 
  * array).  A server owns a device pool of hbm_budget bytes from which every
  * request's output is allocated; a scheduler thread admits requests in
  * arrival order while their outputs fit, and runs each group of admitted
- * requests as one batch launch (the kernels of frsz2_batch_launch) on one of
+ * requests as one batch launch (the grouped kernels of frsz2_batch_launch) on one of
  * `slots` streams, so requests arriving while others run are admitted into the next
  * launch.  A request computes exactly what frsz2_compress / frsz2_decompress
  * / frsz2_gather compute. */
@@ -314,12 +319,11 @@ typedef struct frsz2_server_config
   uint64_t hbm_budget; /* bytes of the device pool (0 = 64 GiB) */
   int slots;           /* concurrent launches (0 = 4) */
   int max_batch;       /* requests per launch (0 = 1024) */
-  int tile_shape;      /* batch kernel tile shape (see frsz2_batch_config); 3
-                          (one grouped launch per class, one workgroup per
-                          tile) gave a 2.2x lower p99 than 0 on a mixed
-                          4 KiB-256 MiB request stream; out of range = 3 */
-  int workgroups;      /* persistent grid (0 = 4 per CU) */
-  int tiles_per_claim; /* 0 = static round-robin */
+  int schedule;        /* tile schedule of the batch launches (see
+                          frsz2_batch_config; 0 = one workgroup per tile) */
+  int workgroups;      /* persistent schedule: workgroups per CU (0 = as many
+                          as the device holds) */
+  int xcd_order;       /* 1 = XCD-contiguous tile order */
   int stall_ms;        /* when nothing runs and the oldest queued request has
                           not fit the pool for this long, it fails with
                           FRSZ2_ECAPACITY (0 = 10000; negative = never) */

@@ -53,16 +53,16 @@ def oracle_outputs(port, specs):
 
 
 CONFIGS = [None,
-           {"workgroups": 64, "tile_shape": 0, "tiles_per_claim": 0, "use_graph": 1},
-           {"workgroups": 1024, "tile_shape": 2, "tiles_per_claim": 1, "use_graph": 0},
-           {"workgroups": 512, "tile_shape": 2, "tiles_per_claim": 16, "use_graph": 1},
+           {"schedule": 1, "use_graph": 1},
+           {"schedule": 1, "workgroups": 1, "use_graph": 0},
+           {"xcd_order": 1, "use_graph": 1},
+           {"schedule": 1, "xcd_order": 1, "workgroups": 2, "use_graph": 1},
            {"plain_launches": 1, "use_graph": 1},
-           {"tile_shape": 3, "use_graph": 1},
-           {"tile_shape": 1, "workgroups": 700, "tiles_per_claim": 3, "use_graph": 1}]
+           {"use_graph": 0}]
 
 
-@pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "static", "nograph", "chunk16", "plain-graph", "grouped",
-                                             "balanced-odd"])
+@pytest.mark.parametrize("cfg", CONFIGS, ids=["default", "persistent", "persistent-1-per-cu-nograph", "xcd",
+                                             "persistent-xcd", "plain-graph", "nograph"])
 def test_batch_matches_oracle(fz, port, cfg):
     fz_mod, batch = fz
     specs = make_arrays(1)
@@ -83,14 +83,14 @@ def test_batch_matches_oracle(fz, port, cfg):
 
 
 @pytest.mark.parametrize("count,cfg", [(1, None), (3, {"use_graph": 0}), (8, {"use_graph": 1}),
-                                       (9, {"use_graph": 0}), (5, {"workgroups": 16, "tiles_per_claim": 2}),
-                                       (4, {"workgroups": 16, "tiles_per_claim": 0, "use_graph": 1}),
-                                       (3, {"tile_shape": 3, "use_graph": 0}), (8, {"tile_shape": 3, "use_graph": 1}),
-                                       (9, {"tile_shape": 3, "use_graph": 1})])
+                                       (9, {"use_graph": 0}), (5, {"schedule": 1}),
+                                       (4, {"schedule": 1, "xcd_order": 1, "use_graph": 1}),
+                                       (3, {"xcd_order": 1, "use_graph": 0}), (8, {"xcd_order": 1, "use_graph": 1}),
+                                       (9, {"xcd_order": 1, "use_graph": 1})])
 def test_few_jobs(fz, port, count, cfg):
-    """Up to eight descriptors over a grid of one workgroup per tile (or the
-    grouped shape) travel in the kernel arguments (no descriptor copy); nine,
-    or a grid smaller than the tile count, read them from device memory."""
+    """Up to eight descriptors of a class travel in the kernel arguments of a
+    one-workgroup-per-tile launch (no descriptor copy); nine, or the
+    persistent schedule, read them from device memory."""
     fz_mod, batch = fz
     specs = make_arrays(30 + count, count=count)
     ref_s, ref_v = oracle_outputs(port, specs)
@@ -114,24 +114,18 @@ def test_few_jobs(fz, port, count, cfg):
     dplan.close()
 
 
-@pytest.mark.parametrize("shape", [0, 2])
-@pytest.mark.parametrize("grid", [1, 5, 6, 11, 12, 13, 64])
-def test_round_robin_grid_edges(fz, port, shape, grid):
-    """Static round-robin (no claims) over a tile table: twelve float32 jobs of
-    one tile each (more than fit in the kernel arguments), with grids that
-    take every tile in one step, leave a remainder, end exactly on a full
-    round, or exceed the tile count.  The job lookups run one and two tiles
-    ahead of the tile in hand, so these are the cases where they reach past
-    the last tile."""
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("count", [1, 5, 8, 12, 13, 64])
+def test_xcd_order_tile_counts(fz, port, schedule, count):
+    """XCD order: a class of `count` one-tile jobs (float32, bits 16, B 32),
+    tile counts below, at and above a multiple of the 8 XCDs, on both
+    schedules; every tile is taken exactly once."""
     fz_mod, batch = fz
     from frsz_amd.workloads import smooth_signal_torch
 
-    specs = []
-    for i in range(12):
-        x = smooth_signal_torch(1000 - 37 * i, seed=500 + i, dtype=torch.float32)
-        specs.append((x, (16, 12, 21, 32)[i % 4], 32))
+    specs = [(smooth_signal_torch(1000 - 7 * i, seed=500 + i, dtype=torch.float32), 16, 32) for i in range(count)]
     ref_s, ref_v = oracle_outputs(port, specs)
-    cfg = {"tile_shape": shape, "workgroups": grid, "tiles_per_claim": 0, "use_graph": 1}
+    cfg = {"schedule": schedule, "xcd_order": 1, "use_graph": 1}
     outs = [torch.full_like(s, 0x5A) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)], cfg)
     vals = [torch.full_like(x, float("nan")) for x, _, _ in specs]
@@ -140,9 +134,9 @@ def test_round_robin_grid_edges(fz, port, shape, grid):
     dplan.launch()
     torch.cuda.synchronize()
     for i, (o, r) in enumerate(zip(outs, ref_s)):
-        assert torch.equal(o, r), f"grid {grid} shape {shape} job {i}"
+        assert torch.equal(o, r), f"{count} jobs schedule {schedule} job {i}"
     for i, (v, r) in enumerate(zip(vals, ref_v)):
-        assert torch.equal(v.view(torch.uint8), r.view(torch.uint8)), f"grid {grid} shape {shape} decompress job {i}"
+        assert torch.equal(v.view(torch.uint8), r.view(torch.uint8)), f"{count} jobs decompress job {i}"
     plan.close()
     dplan.close()
 
@@ -173,8 +167,8 @@ def test_autotune_keeps_bytes(fz, port):
     outs = [torch.empty_like(s) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
     chosen = plan.autotune(reps=1)
-    # whichever candidate wins the timing, the plan keeps a valid shape and the bytes
-    assert chosen["tile_shape"] in (0, 1, 2, 3)
+    # whichever candidate wins the timing, the plan keeps a valid schedule and the bytes
+    assert chosen["schedule"] in (0, 1) and chosen["xcd_order"] in (0, 1)
     assert chosen == plan.config
     for o in outs:
         o.zero_()
@@ -197,13 +191,14 @@ def test_batch_rejects_bad_jobs(fz):
     assert e.value.code == 2 and "Unsupported number of bits: 33" in str(e.value)
 
 
-@pytest.mark.parametrize("cfg", [None, {"tile_shape": 0, "use_graph": 0}, {"tile_shape": 2, "use_graph": 0},
-                                 {"tile_shape": 3, "use_graph": 1}], ids=["default", "narrow", "wide", "grouped"])
+@pytest.mark.parametrize("cfg", [None, {"schedule": 1, "use_graph": 0}, {"xcd_order": 1, "use_graph": 0},
+                                 {"schedule": 1, "xcd_order": 1, "use_graph": 1}],
+                         ids=["default", "persistent", "xcd", "persistent-xcd"])
 def test_decompress_and_gather_of_arbitrary_streams(fz, port, cfg):
     """Streams no compress writes (random payload and block exponents over
     the range where the reference decode is defined, E + bias < 0 included):
     every batched decompress and gather equals the oracle's, for each tile
-    shape."""
+    schedule."""
     fz_mod, batch = fz
     from oracle.oracle import compressed_bytes, storage_bits
 

@@ -37,12 +37,12 @@ def arrays(seed, count):
     return out
 
 
-@pytest.mark.parametrize("tile_shape", [3, 0, 2])
-def test_mix_equals_oracle(mods, port, tile_shape):
+@pytest.mark.parametrize("schedule,xcd_order", [(0, 0), (1, 0), (0, 1)], ids=["tiles", "persistent", "xcd"])
+def test_mix_equals_oracle(mods, port, schedule, xcd_order):
     fz, serve = mods
     specs = arrays(1, 40)
     rng = np.random.default_rng(2)
-    with serve.Server(hbm_budget=1 << 30, slots=3, tile_shape=tile_shape) as srv:
+    with serve.Server(hbm_budget=1 << 30, slots=3, schedule=schedule, xcd_order=xcd_order) as srv:
         comp, dec, gat = [], [], []
         for x, bits, B in specs:
             t = srv.submit_compress(x, bits, B)

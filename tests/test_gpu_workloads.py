@@ -138,7 +138,7 @@ def test_serving_trace_vs_oracle(fz, port_mt):
     rec, idx_all, idx_off = serving_requests(tr, src, B, keep)
     torch.cuda.synchronize()
     print(f"serving: {nreq} requests over 2 s, {int(keep.sum())} checked", flush=True)
-    with serve.Server(hbm_budget=64 << 30, slots=4, tile_shape=3) as srv:
+    with serve.Server(hbm_budget=64 << 30, slots=4) as srv:
         res, tickets = srv.replay(rec)
         assert np.all(res["status"] == 0)
         assert np.all(res["t_arrival"] <= res["t_admit"]) and np.all(res["t_admit"] <= res["t_launch"])
