@@ -7,15 +7,16 @@ in HBM before the timed region starts.  Throughput counts algorithmic bytes:
 compress reads n*8 and writes the stream, decompress the reverse.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--side 512]
-                    [--cpu-sample N] [--no-verify]
+                    [--no-verify] [--no-cpu]
 
 Prints one JSON line.  ``roofline`` is for the dominant kernel (timed with
 HIP events on the stream the kernels run on).  The CPU leg (``cpu_leg``) is
 the only code here that touches the oracle: it checks the GPU outputs of the
-field and of the small case against it (``bit_exact``), and ``cpu_baseline``
-times the reference CPU codec (oracle/_ref, built from the reference sources)
-on a bounded sample of the same field and on the whole small case,
-single-threaded like the reference.
+field, the small case and the edge-case gather against it (``bit_exact``),
+and ``cpu_baseline`` times the reference CPU codec (oracle/_ref, built from
+the reference sources) on one pinned core, best of 5: the whole field, the
+whole small case, every edge-case cell on a 2^24-value sample and the 10^7
+random-access decodes, single-threaded like the reference.
 """
 from __future__ import annotations
 
@@ -41,7 +42,6 @@ def parse():
     ap.add_argument("--side", type=int, default=512)
     ap.add_argument("--bits", type=int, default=32)
     ap.add_argument("--block", type=int, default=32)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 25, help="values in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -147,15 +147,19 @@ def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
     size: compress / decompress GB/s for bits {8,12,16,21,32} x B {16,32,64}
     (best of 3 rounds of `reps` launches), and a uniform random-access decode
     of 10^7 single values (bits 16, B 32).  Every output is compared with the
-    first launch's (determinism)."""
+    first launch's (determinism).  Returns the result and, for the CPU leg,
+    the host values, the bits-16 B-32 stream, the indices and the GPU's
+    gathered values."""
     import numpy as np
     import torch
 
     import frsz_amd
     from frsz_amd.workloads import lognormal_edge
 
-    x = torch.from_numpy(lognormal_edge(n, seed=3)).cuda()
+    x_host = lognormal_edge(n, seed=3)
+    x = torch.from_numpy(x_host).cuda()
     out = {"n": n, "cases": []}
+    host = {"x": x_host}
 
     def timed(fn):
         # best of 3 rounds of `reps` launches, after 2 untimed ones
@@ -189,7 +193,8 @@ def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
                                  "compress_gbs": round(alg / tc / 1e9, 1), "decompress_gbs": round(alg / td / 1e9, 1),
                                  "deterministic": bool(same)})
             if (bits, B) == (16, 32):
-                g = torch.from_numpy(np.random.default_rng(7).integers(0, n, gathers).astype(np.int64)).cuda()
+                g_host = np.random.default_rng(7).integers(0, n, gathers).astype(np.int64)
+                g = torch.from_numpy(g_host).cuda()
                 go = torch.empty(gathers, dtype=torch.float32, device="cuda")
                 from frsz_amd import _capi
 
@@ -205,10 +210,12 @@ def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
                                  "algorithmic_gbs": round(gathers * (8 + 4 + U // 8 + U // 8) / tg / 1e9, 1),
                                  "bound": "transaction (random 64-byte touches), not bandwidth",
                                  "matches_decompress": bool(torch.equal(go.view(torch.int32), ref.view(torch.int32)))}
+                host.update(stream=s.cpu().numpy(), idx=g_host.view(np.uint64), gathered=go.cpu().numpy(),
+                            bits=bits, block=B)
             del s, first
     del x, y
     torch.cuda.empty_cache()
-    return out
+    return out, host
 
 
 def _best_time(fn, reps):
@@ -544,15 +551,63 @@ def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_opti
             "pool_peak_gib": round(stats["pool_peak"] / 2**30, 3), "max_queue": stats["max_queue"]}
 
 
-def cpu_leg(checks, x_host, bits, block, sample, small_values, time_cpu=True):
+def bench_ceiling(mib=1024):
+    """The streaming ceilings of this device from tests/native/stream_ceiling
+    (a child process; plain 16-byte streaming kernels with every read:write
+    mix of the codec): the best GB/s per mix, or None when it is not built."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "native", "stream_ceiling")
+    if not os.path.exists(exe):
+        return None
+    res = subprocess.run([exe, "--mib", str(mib)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                         timeout=300)
+    if res.returncode != 0:
+        return {"error": res.stdout[-500:]}
+    best = {}
+    for ln in res.stdout.splitlines():
+        if ln.startswith("{"):
+            r = json.loads(ln)
+            key = {(1, 0): "read_only", (0, 1): "write_only", (1, 1): "copy_1to1", (2, 1): "read2_write1",
+                   (1, 2): "read1_write2"}[(r["read"], r["write"])]
+            best[key] = max(best.get(key, 0.0), r["gbs"])
+    return {"side_mib": mib, "gbs": best, "source": "tests/native/stream_ceiling.hip, best over workgroup size, "
+            "vectors per lane, non-temporal hint and tile order"}
+
+
+def _best_of(fn, reps=5):
+    """Wall seconds of fn: one untimed warm-up call, then the best of `reps`."""
+    fn()
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    return best
+
+
+def cpu_leg(checks, x_host, bits, block, small_values, edge, time_cpu=True, edge_sample=1 << 24, reps=5):
     """The CPU leg, the one part of this file that touches the oracle.
 
     checks: {name: (values, GPU stream, GPU decode, bits, block)}; each is
     compared byte-for-byte with the oracle (the multithreaded C restatement),
-    giving {name: bool}.  With time_cpu, the reference CPU codec (or the
-    restatement when the reference build is absent) is timed on the first
-    `sample` values of the field (best of 2 passes) and on the whole small
-    case (best of 5), giving the cpu_baseline object (else None)."""
+    giving {name: bool}; with `edge` (bench_edge's host data) the GPU's 10^7
+    gathered values are compared with the oracle's gather as well.
+
+    With time_cpu, the reference CPU codec (the restatement when the
+    reference build is absent) runs on one thread pinned to one core (the
+    calling thread's affinity narrowed to its lowest allowed CPU, restored
+    afterwards), each workload after one untimed warm-up as the best of
+    `reps`, into buffers allocated beforehand:
+      * the float32 2^20-value smooth signal, bits 16, B 32;
+      * the whole float64 512^3 field at (bits, block): the cpu_baseline
+        value is its compress+decompress GB/s;
+      * each (bits, B) cell of the edge-case sweep on its first
+        `edge_sample` values, and the 10^7-index random-access decode
+        (element decode) over the whole 10^8-value stream;
+    and then the block-parallel restatement on every allowed core over the
+    field.  Returns (verdicts, cpu_baseline or None)."""
     import numpy as np
 
     from oracle.oracle import Port, Reference, compressed_bytes, reference_available
@@ -565,56 +620,70 @@ def cpu_leg(checks, x_host, bits, block, sample, small_values, time_cpu=True):
         verdicts[name] = bool(np.array_equal(got_s, want) and
                               np.array_equal(got_y.view(ui), port.decompress(want, vals.size, vals.dtype, b,
                                                                               blk).view(ui)))
+    if edge is not None and "stream" in edge:
+        ex = edge["x"]
+        want = port.gather(edge["stream"], ex.size, ex.dtype, edge["bits"], edge["block"], edge["idx"])
+        verdicts["edge_gather"] = bool(np.array_equal(port.compress(ex, edge["bits"], edge["block"]), edge["stream"])
+                                       and np.array_equal(want.view(np.uint32), edge["gathered"].view(np.uint32)))
     if not time_cpu:
         return verdicts, None
-    xs = np.ascontiguousarray(x_host[:sample])
     kind = "reference" if reference_available() else "port"
     codec = Reference() if kind == "reference" else Port()
-    tc, td = [], []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        s = codec.compress(xs, bits, block)
-        t1 = time.perf_counter()
-        codec.decompress(s, xs.size, xs.dtype, bits, block)
-        t2 = time.perf_counter()
-        tc.append(t1 - t0)
-        td.append(t2 - t1)
-    nbytes = xs.nbytes + compressed_bytes(bits, block, xs.size)
-    tcb, tdb = min(tc), min(td)
-    res = {
-        "value": round(2 * nbytes / (tcb + tdb) / 1e9, 4),
-        "unit": "GB/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": f"first {xs.size} values ({xs.nbytes / 2**20:.0f} MiB) of the same field, compress+decompress, "
-                  f"best of 2, single thread",
-        "compress_gbs": round(nbytes / tcb / 1e9, 4),
-        "decompress_gbs": round(nbytes / tdb / 1e9, 4),
-        "cpu_model": cpu_model(),
-        "nproc": os.cpu_count(),
-    }
-    # the block-parallel restatement on all host cores (same bytes, OpenMP)
-    t0 = time.perf_counter()
-    s = port.compress(xs, bits, block)
-    t1 = time.perf_counter()
-    port.decompress(s, xs.size, xs.dtype, bits, block)
-    t2 = time.perf_counter()
-    res["port_all_cores"] = {"threads": port.max_threads(), "value": round(2 * nbytes / (t2 - t0) / 1e9, 4),
-                             "unit": "GB/s"}
-    if small_values is not None:
-        # the small case whole, single-threaded (the reference's own test size)
-        sb, sblk = 16, 32
-        sc, sd = [], []
-        for _ in range(5):
-            t0 = time.perf_counter()
-            s = codec.compress(small_values, sb, sblk)
-            t1 = time.perf_counter()
-            codec.decompress(s, small_values.size, small_values.dtype, sb, sblk)
-            t2 = time.perf_counter()
-            sc.append(t1 - t0)
-            sd.append(t2 - t1)
-        res["small_case"] = {"values": int(small_values.size), "bits": sb, "block": sblk, "kind": kind,
-                             "compress_us": round(min(sc) * 1e6, 1), "decompress_us": round(min(sd) * 1e6, 1)}
+
+    def directions(vals, b, blk):
+        # (compress seconds, decompress seconds, algorithmic bytes) of one array
+        stream = np.zeros(compressed_bytes(b, blk, vals.size), dtype=np.uint8)
+        back = np.zeros_like(vals)
+        tc = _best_of(lambda: codec.compress_into(vals, stream, b, blk), reps)
+        td = _best_of(lambda: codec.decompress_into(stream, back, b, blk), reps)
+        return tc, td, vals.nbytes + stream.size
+
+    allowed = os.sched_getaffinity(0)
+    core = min(allowed)
+    os.sched_setaffinity(0, {core})
+    try:
+        res = {"unit": "GB/s", "cores": 1, "kind": kind, "pinned_cpu": core}
+        if small_values is not None:
+            tc, td, nb = directions(small_values, 16, 32)
+            res["small_case"] = {"values": int(small_values.size), "bits": 16, "block": 32,
+                                 "compress_us": round(tc * 1e6, 1), "decompress_us": round(td * 1e6, 1),
+                                 "compress_gbs": round(nb / tc / 1e9, 3), "decompress_gbs": round(nb / td / 1e9, 3)}
+        xf = np.ascontiguousarray(x_host)
+        tc, td, nb_field = directions(xf, bits, block)
+        res.update(value=round(2 * nb_field / (tc + td) / 1e9, 4), compress_gbs=round(nb_field / tc / 1e9, 4),
+                   decompress_gbs=round(nb_field / td / 1e9, 4), field_compress_s=round(tc, 4),
+                   field_decompress_s=round(td, 4))
+        res["sample"] = (f"the whole float64 {xf.size}-value field ({xf.nbytes / 2**30:.2f} GiB), bits {bits}, "
+                         f"B {block}: compress+decompress, warm-up then best of {reps}, one thread pinned to CPU "
+                         f"{core}; edge cells on the first {edge_sample} of 10^8 values")
+        if edge is not None:
+            ex = np.ascontiguousarray(edge["x"][:edge_sample])
+            cells = []
+            for b in (8, 12, 16, 21, 32):
+                for blk in (16, 32, 64):
+                    tc, td, nb = directions(ex, b, blk)
+                    cells.append({"bits": b, "block": blk, "compress_gbs": round(nb / tc / 1e9, 3),
+                                  "decompress_gbs": round(nb / td / 1e9, 3)})
+            res["edge_cells"] = {"values": int(ex.size), "cells": cells}
+            if "stream" in edge:
+                ex = edge["x"]
+                tg = _best_of(lambda: codec.gather(edge["stream"], ex.size, ex.dtype, edge["bits"], edge["block"],
+                                                   edge["idx"]), reps)
+                res["edge_gather"] = {"indices": int(edge["idx"].size), "values": int(ex.size),
+                                      "bits": edge["bits"], "block": edge["block"], "ms": round(tg * 1e3, 1),
+                                      "melem_s": round(edge["idx"].size / tg / 1e6, 2)}
+    finally:
+        os.sched_setaffinity(0, allowed)
+    # the block-parallel restatement on every allowed core (same bytes)
+    stream = np.zeros(compressed_bytes(bits, block, xf.size), dtype=np.uint8)
+    back = np.zeros_like(xf)
+    tc = _best_of(lambda: port.compress_into(xf, stream, bits, block), 2)
+    td = _best_of(lambda: port.decompress_into(stream, back, bits, block), 2)
+    res["port_all_cores"] = {"threads": port.max_threads(), "value": round(2 * nb_field / (tc + td) / 1e9, 4),
+                             "unit": "GB/s", "best_of": 2}
+    res["cpu_model"] = cpu_model()
+    res["nproc"] = os.cpu_count()
+    res["allowed_cpus"] = len(allowed)
     return verdicts, res
 
 
@@ -751,12 +820,25 @@ def main():
         line["copy_calibration"] = {"bytes_read": a.numel() * 8, "bytes_written": a.numel() * 8,
                                     "us": round(tcopy * 1e6, 1), "gbs": round(2 * a.numel() * 8 / tcopy / 1e9, 1)}
         del a, b
+    if not args.no_copy:
+        print("bench: streaming ceilings", file=sys.stderr, flush=True)
+        ceil = bench_ceiling()
+        if ceil is not None:
+            line["stream_ceiling"] = ceil
+            g = ceil.get("gbs", {})
+            # float64 at 32 bits: compress reads two bytes per byte written, decompress the reverse
+            mix = {"compress": "read2_write1", "decompress": "read1_write2"}[dom] if (bits, block) == (32, 32) else None
+            if mix in g:
+                line["roofline"]["ceiling_mix"] = mix
+                line["roofline"]["ceiling_gbs"] = g[mix]
+                line["roofline"]["frac_of_ceiling"] = round(achieved / g[mix], 4)
     if not args.no_host:
         print("bench: host-resident path", file=sys.stderr, flush=True)
         line["host_path"] = bench_host(x_host, bits, block, s_host, y_host)
     del x, y, stream
     torch.cuda.empty_cache()
     small_check = None
+    edge_host = None
     if not args.no_small:
         print("bench: small case", file=sys.stderr, flush=True)
         line["small_case"], small_check = bench_small()
@@ -767,7 +849,7 @@ def main():
         line["batched_4gib"] = bench_batched(args.batched_arrays, args.batched_steps, target_input_bytes=1 << 32)
     if not args.no_edge:
         print("bench: edge-case sweep", file=sys.stderr, flush=True)
-        line["edge"] = bench_edge()
+        line["edge"], edge_host = bench_edge()
     if not args.no_generic:
         print("bench: block sizes that are not powers of two, unaligned streams", file=sys.stderr, flush=True)
         line["generic"] = bench_generic()
@@ -780,12 +862,14 @@ def main():
             checks["field"] = (x_host, s_host, y_host, bits, block)
             if small_check is not None:
                 checks["small"] = small_check
-        verdicts, cpu = cpu_leg(checks, x_host, bits, block, min(args.cpu_sample, n),
-                                None if small_check is None else small_check[0], time_cpu=not args.no_cpu)
+        verdicts, cpu = cpu_leg(checks, x_host, bits, block, None if small_check is None else small_check[0],
+                                edge_host, time_cpu=not args.no_cpu)
         if "field" in verdicts:
             line["bit_exact"] = verdicts["field"]
         if "small" in verdicts:
             line["small_case"]["bit_exact"] = verdicts["small"]
+        if "edge_gather" in verdicts:
+            line["edge"]["gather"]["bit_exact"] = verdicts["edge_gather"]
         if cpu is not None:
             line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

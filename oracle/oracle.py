@@ -90,6 +90,24 @@ class _Codec:
             raise RuntimeError(f"{self.prefix}decompress rc={rc} (bits={bits}, B={B})")
         return out
 
+    def compress_into(self, x: np.ndarray, out: np.ndarray, bits: int, B: int) -> None:
+        """compress into a caller-owned contiguous uint8 buffer of at least
+        compressed_bytes(bits, B, x.size) bytes (nothing allocated: for
+        timing the codec alone)."""
+        assert x.flags.c_contiguous and out.flags.c_contiguous and out.dtype == np.uint8
+        assert out.size >= compressed_bytes(bits, B, x.size)
+        rc = self._compress(dtype_code(x.dtype), bits, B, x, out)
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}compress rc={rc} (bits={bits}, B={B})")
+
+    def decompress_into(self, stream: np.ndarray, out: np.ndarray, bits: int, B: int) -> None:
+        """decompress out.size values into a caller-owned contiguous array."""
+        assert stream.flags.c_contiguous and out.flags.c_contiguous and stream.dtype == np.uint8
+        assert stream.size >= compressed_bytes(bits, B, out.size)
+        rc = self._decompress(dtype_code(out.dtype), bits, B, stream, out.size, out)
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}decompress rc={rc} (bits={bits}, B={B})")
+
     def gather(self, stream: np.ndarray, n: int, dtype, bits: int, B: int, idx: np.ndarray) -> np.ndarray:
         code = dtype_code(dtype)
         idx = np.ascontiguousarray(idx, dtype=np.uint64)

@@ -11,7 +11,9 @@
   tests/test_plugin.py and tests/test_gpu_plugin.py;
 * test_header_api: the C++ header-API program (include/frsz2.hpp) that
   tests/test_gpu_header_api.py runs, linked against frsz_amd/lib/libfrsz2_hip.so
-  (build that first).
+  (build that first);
+* stream_ceiling: the HBM streaming ceilings (read-only, write-only, 1:1,
+  2:1, 1:2 read:write) the codec's kernels are measured against.
 
     python tests/native/build.py
 """
@@ -55,6 +57,8 @@ def build_test_helpers() -> None:
           "-packed-fp32-ops", f"-I{INCLUDE}", "-o", os.path.join(HERE, "test_header_api"),
           os.path.join(HERE, "test_header_api.cpp"), f"-L{LIBDIR}", "-lfrsz2_hip", f"-Wl,-rpath,{LIBDIR}",
           "-Wl,-rpath,$ORIGIN/../../frsz_amd/lib"])
+    _run([hipcc, "--offload-arch=gfx950", "-std=c++17", "-O3", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-o", os.path.join(HERE, "stream_ceiling"), os.path.join(HERE, "stream_ceiling.hip")])
 
 
 if __name__ == "__main__":

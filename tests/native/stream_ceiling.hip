@@ -1,0 +1,300 @@
+// HBM streaming ceilings of the device: what plain coalesced 16-byte loads
+// and stores reach for the read:write mixes the codec's kernels have.
+//
+//   read-only (1:0), write-only (0:1), copy (1:1),
+//   2:1 (compress of float64 at 32 bits reads 8 bytes per 4 written),
+//   1:2 (the matching decompress).
+//
+// Every workgroup streams one tile: it reads R units and writes W units of
+// blockDim.x * UNR 16-byte vectors, each lane loading all of its vectors
+// before it stores any.  The tiles cover the buffers exactly (the sizes are
+// multiples of the largest tile), so no lane is out of bounds.  Read-only
+// tiles store only when a lane's XOR of its vectors equals a key the fill
+// pattern never produces, into a sink of blockDim.x vectors.  Variants:
+// plain or non-temporal loads and stores (NT), and workgroup order or one
+// contiguous range of tiles per XCD (workgroup b runs on XCD b mod 8), the
+// two choices the codec's tile kernels make.
+//
+// Usage: stream_ceiling [--mib N]   (N MiB per side, default 1024)
+// Prints one JSON line per (pattern, workgroup size, UNR, NT, order) with
+// the best of three rounds of ten launches, event-timed, and exits non-zero
+// if a copy's or a mix's bytes are wrong.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#define HIP_CHECK(expr)                                                                                     \
+  do {                                                                                                      \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess) {                                                                                 \
+      std::fprintf(stderr, "%s failed: %s\n", #expr, hipGetErrorString(e_));                               \
+      std::exit(2);                                                                                         \
+    }                                                                                                       \
+  } while (0)
+
+namespace {
+
+__device__ __forceinline__ uint4
+xor4(uint4 a, uint4 b)
+{
+  return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
+}
+
+__device__ __forceinline__ uint4
+pattern(std::uint64_t i)
+{
+  const auto lo = static_cast<std::uint32_t>(i);
+  return make_uint4(lo * 0x9E3779B1u, lo, ~lo, lo ^ 0x55AA55AAu);
+}
+
+__global__ void
+k_fill(uint4* p, std::uint64_t count)
+{
+  for (std::uint64_t i = blockIdx.x * std::uint64_t(blockDim.x) + threadIdx.x; i < count;
+       i += std::uint64_t(gridDim.x) * blockDim.x)
+    p[i] = pattern(i);
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template<bool NT>
+__device__ __forceinline__ uint4
+load(const uint4* p)
+{
+  if constexpr (NT) {
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+  } else {
+    return *p;
+  }
+}
+
+template<bool NT>
+__device__ __forceinline__ void
+store(uint4 v, uint4* p)
+{
+  if constexpr (NT) {
+    const u32x4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<u32x4*>(p));
+  } else {
+    *p = v;
+  }
+}
+
+// tile of workgroup b: with xcd order, the tiles are split into 8 contiguous
+// ranges (the first ntiles % 8 one longer) and XCD b % 8 takes range b % 8
+__device__ __forceinline__ std::uint32_t
+tile_of(std::uint32_t b, std::uint32_t ntiles, bool xcd)
+{
+  if (!xcd)
+    return b;
+  const std::uint32_t x = b & 7u, local = b >> 3, q = ntiles >> 3, r = ntiles & 7u;
+  return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
+// R units read and W units written per tile; a unit is blockDim.x * UNR vectors
+template<int R, int W, int UNR, bool NT>
+__global__ void
+k_stream(const uint4* __restrict__ src, uint4* __restrict__ dst, uint4* __restrict__ sink, uint4 key, int xcd)
+{
+  const std::uint64_t unit = std::uint64_t(blockDim.x) * UNR;
+  const std::uint64_t t = tile_of(blockIdx.x, gridDim.x, xcd != 0);
+  const unsigned lane = threadIdx.x;
+  uint4 v[(R > 0 ? R : 1) * UNR];
+  if constexpr (R > 0) {
+    const uint4* s = src + t * R * unit;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
+        v[r * UNR + u] = load<NT>(s + r * unit + u * blockDim.x + lane);
+  } else {
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      v[u] = make_uint4(static_cast<unsigned>(t), lane, u, 0x5A5A5A5Au);
+  }
+  if constexpr (W == 0) {
+    uint4 acc = v[0];
+#pragma unroll
+    for (int k = 1; k < R * UNR; ++k)
+      acc = xor4(acc, v[k]);
+    if (acc.x == key.x && acc.y == key.y && acc.z == key.z && acc.w == key.w)
+      sink[lane] = acc;
+  } else {
+    uint4* d = dst + t * W * unit;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        uint4 o = v[u];
+        if constexpr (R == 2)
+          o = xor4(v[u], v[UNR + u]);
+        if (w == 1)
+          o = xor4(o, make_uint4(1u, 2u, 3u, 4u));
+        store<NT>(o, d + w * unit + u * blockDim.x + lane);
+      }
+  }
+}
+
+struct result
+{
+  double us;
+  double gbs;
+};
+
+template<int R, int W, int UNR, bool NT>
+result
+run(const uint4* src, uint4* dst, uint4* sink, std::uint64_t side_bytes, int wg, int xcd, hipEvent_t e0,
+    hipEvent_t e1)
+{
+  const std::uint64_t unit_bytes = std::uint64_t(wg) * UNR * 16;
+  const int m = R > W ? R : W;
+  const std::uint64_t tiles = side_bytes / (unit_bytes * m);
+  const uint4 key = make_uint4(0xDEADBEEFu, 0x01234567u, 0x89ABCDEFu, 0xFEDCBA98u);
+  auto launch = [&] {
+    hipLaunchKernelGGL((k_stream<R, W, UNR, NT>), dim3(static_cast<unsigned>(tiles)), dim3(wg), 0, nullptr, src,
+                       dst, sink, key, xcd);
+  };
+  for (int i = 0; i < 2; ++i)
+    launch();
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  float best = 0.f;
+  for (int round = 0; round < 3; ++round) {
+    HIP_CHECK(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < 10; ++i)
+      launch();
+    HIP_CHECK(hipEventRecord(e1, nullptr));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (round == 0 || ms < best)
+      best = ms;
+  }
+  const double us = best / 10.0 * 1e3;
+  const double bytes = double(tiles) * unit_bytes * (R + W);
+  return {us, bytes / (us * 1e-6) / 1e9};
+}
+
+// host copy of the vectors [first, first + count) of the fill pattern
+std::vector<uint4>
+host_pattern(std::uint64_t first, std::uint64_t count)
+{
+  std::vector<uint4> h(count);
+  for (std::uint64_t k = 0; k < count; ++k) {
+    const auto lo = static_cast<std::uint32_t>(first + k);
+    h[k] = make_uint4(lo * 0x9E3779B1u, lo, ~lo, lo ^ 0x55AA55AAu);
+  }
+  return h;
+}
+
+bool
+same(const uint4& a, const uint4& b)
+{
+  return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+}
+
+// checks the first `count` vectors of dst after a pattern (R, W) ran with
+// workgroup size wg and UNR (the first tile holds them when count <= unit)
+bool
+check_tile(const uint4* dst, int R, int W, int wg, int unr)
+{
+  const std::uint64_t unit = std::uint64_t(wg) * unr;
+  std::vector<uint4> got(unit * W);
+  HIP_CHECK(hipMemcpy(got.data(), dst, got.size() * 16, hipMemcpyDeviceToHost));
+  const auto s0 = host_pattern(0, unit);
+  const auto s1 = host_pattern(unit, unit);
+  for (int w = 0; w < W; ++w)
+    for (std::uint64_t k = 0; k < unit; ++k) {
+      uint4 want;
+      if (R == 0) {
+        const unsigned lane = static_cast<unsigned>(k % wg), u = static_cast<unsigned>(k / wg);
+        want = make_uint4(0u, lane, u, 0x5A5A5A5Au);
+      } else if (R == 1) {
+        want = s0[k];
+      } else {
+        want = make_uint4(s0[k].x ^ s1[k].x, s0[k].y ^ s1[k].y, s0[k].z ^ s1[k].z, s0[k].w ^ s1[k].w);
+      }
+      if (w == 1)
+        want = make_uint4(want.x ^ 1u, want.y ^ 2u, want.z ^ 3u, want.w ^ 4u);
+      if (!same(got[w * unit + k], want)) {
+        std::printf("FAIL pattern %d:%d wg %d unr %d vector %llu\n", R, W, wg, unr,
+                    static_cast<unsigned long long>(w * unit + k));
+        return false;
+      }
+    }
+  return true;
+}
+
+} // namespace
+
+int
+main(int argc, char** argv)
+{
+  std::uint64_t mib = 1024;
+  for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--mib") && i + 1 < argc)
+      mib = std::strtoull(argv[++i], nullptr, 10);
+  }
+  // the largest tile is 2 units of 256 * 4 vectors (32 KiB): sizes in MiB divide evenly
+  if (mib < 1 || mib > 16384) {
+    std::fprintf(stderr, "--mib must be in 1..16384\n");
+    return 2;
+  }
+  const std::uint64_t side = mib << 20;
+  uint4 *src = nullptr, *dst = nullptr, *sink = nullptr;
+  HIP_CHECK(hipMalloc(&src, side));
+  HIP_CHECK(hipMalloc(&dst, side));
+  HIP_CHECK(hipMalloc(&sink, 256 * 16));
+  hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, nullptr, src, side / 16);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  int fails = 0;
+  auto report = [&](const char* name, int R, int W, int wg, int unr, int nt, int xcd, result r) {
+    std::printf("{\"pattern\": \"%s\", \"read\": %d, \"write\": %d, \"workgroup\": %d, \"vectors_per_lane\": %d, "
+                "\"nt\": %d, \"xcd_order\": %d, \"side_mib\": %llu, \"us\": %.1f, \"gbs\": %.1f}\n",
+                name, R, W, wg, unr, nt, xcd, static_cast<unsigned long long>(mib), r.us, r.gbs);
+    std::fflush(stdout);
+    if (W > 0 && !check_tile(dst, R, W, wg, unr))
+      ++fails;
+  };
+#define CEIL_RUN(name, R, W, UNR, NT)                                                                      \
+  report(name, R, W, wg, UNR, NT, xcd, run<R, W, UNR, NT>(src, dst, sink, side, wg, xcd, e0, e1))
+  for (int wg : {256, 64})
+    for (int xcd : {0, 1}) {
+      CEIL_RUN("read-only", 1, 0, 4, false);
+      CEIL_RUN("read-only", 1, 0, 1, false);
+      CEIL_RUN("read-only", 1, 0, 4, true);
+      CEIL_RUN("read-only", 1, 0, 1, true);
+      CEIL_RUN("write-only", 0, 1, 4, false);
+      CEIL_RUN("write-only", 0, 1, 1, false);
+      CEIL_RUN("write-only", 0, 1, 4, true);
+      CEIL_RUN("write-only", 0, 1, 1, true);
+      CEIL_RUN("copy 1:1", 1, 1, 4, false);
+      CEIL_RUN("copy 1:1", 1, 1, 1, false);
+      CEIL_RUN("copy 1:1", 1, 1, 4, true);
+      CEIL_RUN("copy 1:1", 1, 1, 1, true);
+      CEIL_RUN("2:1", 2, 1, 4, false);
+      CEIL_RUN("2:1", 2, 1, 1, false);
+      CEIL_RUN("2:1", 2, 1, 4, true);
+      CEIL_RUN("2:1", 2, 1, 1, true);
+      CEIL_RUN("1:2", 1, 2, 4, false);
+      CEIL_RUN("1:2", 1, 2, 1, false);
+      CEIL_RUN("1:2", 1, 2, 4, true);
+      CEIL_RUN("1:2", 1, 2, 1, true);
+    }
+#undef CEIL_RUN
+  HIP_CHECK(hipFree(src));
+  HIP_CHECK(hipFree(dst));
+  HIP_CHECK(hipFree(sink));
+  if (fails)
+    std::printf("FAIL %d pattern checks\n", fails);
+  return fails ? 1 : 0;
+}
