@@ -133,6 +133,12 @@ def bench_batched(count, steps, seed=4, target_input_bytes=None):
         "speedup_compress": round(t_pc / t_bc, 2), "speedup_decompress": round(t_pd / t_bd, 2),
         "bit_exact_vs_plain": bool(same_c and same_d),
         "autotuned": {"compress": tuned_c, "decompress": tuned_d},
+        # the most any batched launch could gain over this plain loop: the
+        # loop's time over the time of these bytes at the HBM peak
+        "speedup_ceiling_at_hbm_peak": {"compress": round(t_pc / (alg / (HBM_PEAK_GBS * 1e9)), 2),
+                                        "decompress": round(t_pd / (alg / (HBM_PEAK_GBS * 1e9)), 2)},
+        "bound_note": "the plain loop moves most bytes at full bandwidth on its large arrays; the batch removes "
+                      "the per-launch dispatch gaps, so the speedup cannot exceed speedup_ceiling_at_hbm_peak",
     })
     for p_ in (plain_c, plain_d, plain_cg, plan_c, plan_d):
         p_.close()

@@ -336,6 +336,37 @@ struct frsz2_compressor
     return frsz2::decode_element<fp_type, uint_compressed_size_bit>(data_, L, idx);
   }
 
+  // decompress_gpu_value, include/frsz2.hpp:665-700: the value of one code
+  // (its low bits_per_value bits; higher bits are ignored) in an exponent
+  // block of the given exponent, decoded as the host path decodes it (a zero
+  // code gives +-2^(exponent - 31), include/frsz2.hpp:1199-1214).  Callable
+  // on the host and in kernels, for every bits_per_value.
+  static FRSZ2_HD fp_type decompress_gpu_value(const exp_type exponent, const uint_compressed_type extracted_value)
+  {
+    using uw = typename frsz2::fmt<fp_type>::uw;
+    return __builtin_bit_cast(
+      fp_type, frsz2::decode_value<fp_type>(static_cast<uw>(extracted_value), static_cast<int>(exponent), bits_per_value));
+  }
+
+  // decompress_gpu_element_impl, include/frsz2.hpp:646-775: value idx given
+  // the payload words (block k starts at word k *
+  // compressed_block_size_element_count, as get_compressed_data() lays them
+  // out) and the exponent slots (get_exponent_data()).  Reads exponent slot
+  // idx / B and the one or two words holding the code, nothing past the
+  // block.  Callable on the host and in kernels.
+  template<typename IndexType>
+  static FRSZ2_HD fp_type decompress_gpu_element_impl(const uint_compressed_type* __restrict__ compressed,
+                                                      const exp_type* __restrict__ exponents, const IndexType idx)
+  {
+    const std::uint64_t i = static_cast<std::uint64_t>(idx);
+    const std::uint64_t k = i / static_cast<std::uint64_t>(max_exp_block_size);
+    const std::uint64_t j = i - k * static_cast<std::uint64_t>(max_exp_block_size);
+    const auto* blk = reinterpret_cast<const std::uint8_t*>(
+      compressed + k * static_cast<std::uint64_t>(compressed_block_size_element_count));
+    const std::uint64_t code = frsz2::load_code<uint_compressed_size_bit>(blk, j, bits_per_value);
+    return decompress_gpu_value(exponents[k], static_cast<uint_compressed_type>(code));
+  }
+
   // compress_cpu_block, include/frsz2.hpp:1108-1179: block exp_block_id from
   // its first exp_block_elements values (the rest of the block's codes are
   // written as zero)
@@ -434,6 +465,35 @@ struct frsz2_compressor
     const std::size_t count = total_elements_ - first < max_exp_block_size ? total_elements_ - first
                                                                            : max_exp_block_size;
     frsz2::decode_block<fp_type, uint_compressed_size_bit>(data_, L, exp_block_id, output, count);
+  }
+
+  // general_compress_gpu_function, include/frsz2.hpp:797-914: exponent
+  // block exponent_block_idx compressed by one 1-D workgroup of at least B
+  // threads (B <= 1024), thread t passing value t of the block; every thread
+  // calls it with the same exponent_block_idx.  Threads at or past B, and
+  // values at or past n, are ignored.  Writes the block's exponent slot and
+  // all of its payload bytes in the stream format, the bytes
+  // compress_cpu_block writes.  (The reference's version writes the exponent
+  // in front of the block's payload, a layout its decoders do not read.)
+  __device__ void general_compress_gpu_function(const std::size_t exponent_block_idx, const fp_type fp_input_value)
+  {
+    static_assert(max_exp_block_size <= 1024, "one workgroup holds the exponent block");
+    using shape = detail::wg_shape<frsz2_compressor, max_exp_block_size>;
+    using uw = typename frsz2::fmt<fp_type>::uw;
+    __shared__ uw vals[max_exp_block_size];
+    __shared__ std::uint32_t keys[shape::seg];
+    __shared__ std::uint32_t image[shape::image_dwords];
+    if (threadIdx.x < max_exp_block_size) {
+      vals[threadIdx.x] = __builtin_bit_cast(uw, fp_input_value);
+    }
+    __syncthreads();
+    const frsz2::layout L = frsz2::layout::make(bits_per_value, max_exp_block_size, total_elements_);
+    if (exponent_block_idx < L.nb) {
+      const std::size_t first = exponent_block_idx * max_exp_block_size;
+      detail::wg_compress_step<frsz2_compressor, max_exp_block_size>(
+        data_, L, exponent_block_idx, [&](std::size_t i) { return vals[i - first]; }, keys, image);
+    }
+    __syncthreads(); // vals is reused by the workgroup's next call
   }
 
   // zero the exponent pad (byte i of it, for i = lane, lane + lanes, ...)

@@ -13,8 +13,11 @@
 //      overloads), taken back to the code by uint_fp_to_compressed and
 //      decoded with the block's exponent, is y[i]; and re-packing every code
 //      with compressed_to_uint_fp + write_shift_value into zeroed words gives
-//      the stream's payload bytes.
-// Returns 0, or 1 + the index of the first failed check in *where.
+//      the stream's payload bytes;
+//   3. decompress_gpu_value(exponent, code) and
+//      decompress_gpu_element_impl(payload words, exponents, i) are y[i].
+// Returns 0 or the number of the failed check (1-5), with the index of the
+// first failing value in *where.
 #include "frsz2.hpp"
 
 #include <cstdint>
@@ -71,6 +74,12 @@ check_case(const std::uint8_t* stream, std::uint64_t n, const void* y_raw, std::
     if (v != y[i]) {
       *where = i;
       return 3;
+    }
+    const T gv = C::decompress_gpu_value(exps[k], code);
+    const T ge = C::decompress_gpu_element_impl(words, exps, i);
+    if (std::memcmp(&gv, &y[i], sizeof(T)) != 0 || std::memcmp(&ge, &y[i], sizeof(T)) != 0) {
+      *where = i;
+      return 5;
     }
     word_t first{ 0 };
     word_t second{ 0 };

@@ -88,6 +88,33 @@ __global__ __launch_bounds__(BS) void function_kernel(C view, const typename C::
   }
 }
 
+// general_compress_gpu_function: one workgroup per exponent block, thread t
+// passing value t of the block; workgroup 0 also clears the exponent pad
+template<class C>
+__global__ void
+general_kernel(C view, const typename C::fp_type* in, std::size_t n)
+{
+  constexpr std::size_t B = C::max_exp_block_size;
+  if (blockIdx.x == 0) {
+    view.clear_exponent_pad(threadIdx.x, blockDim.x);
+  }
+  const std::size_t k = blockIdx.x;
+  const std::size_t i = k * B + threadIdx.x;
+  view.general_compress_gpu_function(k, threadIdx.x < B && i < n ? in[i] : typename C::fp_type{});
+}
+
+// the static decoders in a kernel: value i from the view's payload words and
+// exponent slots
+template<class C>
+__global__ void
+impl_kernel(C view, std::size_t n, typename C::fp_type* out)
+{
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[i] = C::decompress_gpu_element_impl(view.get_const_compressed_data(), view.get_const_exponent_data(), i);
+  }
+}
+
 // the reference header's free kernels, launched as the reference's GPU test
 // launches them (test/test_frsz_gpu.cu:325-341, 369-370) and with other
 // grids (grid-stride loops, fewer threads than block_size), into buffers of
@@ -133,6 +160,27 @@ check_free_kernels(C dev_view, const typename C::fp_type* d_in, typename C::fp_t
     CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "%s b%d B%d: decompress_gpu<<<%u,%u>>> differs",
           tname, bits, B, g.g, g.t);
   }
+  // general_compress_gpu_function: one workgroup of B threads (or B rounded
+  // up to a whole wave) per exponent block
+  if constexpr (B <= 1024) {
+    const unsigned nb = static_cast<unsigned>((n + B - 1) / B);
+    for (unsigned t : { static_cast<unsigned>(B), static_cast<unsigned>((B + 63) / 64 * 64) }) {
+      HIP_OK(hipMemset(d_stream, 0x5A, size));
+      general_kernel<C><<<dim3(nb), dim3(t)>>>(dev_view, d_in, n);
+      HIP_OK(hipDeviceSynchronize());
+      HIP_OK(hipMemcpy(dstream.data(), d_stream, size, hipMemcpyDeviceToHost));
+      CHECK(dstream == stream, "%s b%d B%d: general_compress_gpu_function (%u threads) stream differs", tname, bits,
+            B, t);
+    }
+  }
+  // decompress_gpu_element_impl in a kernel
+  HIP_OK(hipMemcpy(d_stream, stream.data(), size, hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(d_out, 0x7F, n * sizeof(T)));
+  impl_kernel<C><<<dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256)>>>(dev_view, n, d_out);
+  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipMemcpy(dout.data(), d_out, n * sizeof(T), hipMemcpyDeviceToHost));
+  CHECK(std::memcmp(dout.data(), out.data(), n * sizeof(T)) == 0, "%s b%d B%d: decompress_gpu_element_impl differs",
+        tname, bits, B);
   // compress_gpu_function in a user kernel, workgroups of 4B threads
   for (unsigned g : { static_cast<unsigned>((n + comp_threads - 1) / comp_threads), 2u }) {
     HIP_OK(hipMemset(d_stream, 0xC3, size));

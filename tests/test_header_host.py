@@ -2,8 +2,9 @@
 program tests/native/header_host.cpp, compiled with g++, checks the view's
 host-side element decode and the reference's public static helpers
 (compressed_to_uint_fp / uint_fp_to_compressed / write_shift_value /
-retrieve_shift_value, include/frsz2.hpp:550-644) on every golden vector the
-reference codec wrote."""
+retrieve_shift_value, include/frsz2.hpp:550-644) and the static decoders
+decompress_gpu_value / decompress_gpu_element_impl (:646-775, callable on the
+host here) on every golden vector the reference codec wrote."""
 import ctypes
 import os
 
@@ -30,7 +31,8 @@ def hh():
 
 
 WHAT = {1: "decompress_cpu_element", 2: "retrieve_shift_value overloads disagree",
-        3: "retrieve_shift_value code decodes to another value", 4: "write_shift_value repack differs"}
+        3: "retrieve_shift_value code decodes to another value", 4: "write_shift_value repack differs",
+        5: "decompress_gpu_value / decompress_gpu_element_impl differ from the decode"}
 
 
 @pytest.mark.parametrize("case", GOLDEN_CASES, ids=repr)
