@@ -56,6 +56,10 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=20000.0)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--serve-load-rates", default="100000,200000",
+                    help="comma-separated request rates of the short load runs after the serving workload "
+                         "('' for none)")
+    ap.add_argument("--serve-load-seconds", type=float, default=3.0)
     return ap.parse_args()
 
 
@@ -807,7 +811,10 @@ def main():
         "bit_exact": None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(dom, side, bits, block), "algorithmic_bytes": alg},
+                     "traffic": profiled_traffic(dom, side, bits, block), "algorithmic_bytes": alg,
+                     "traffic_source": "not measured in this run: HBM bytes per launch of this kernel on this "
+                                       "workload from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary "
+                                       "profiles/plain_f64_512cubed_b32_B32_hbm_traffic.md (x2.0 / x1.0)"},
     }
     if not args.no_copy:
         # a plain 1 GiB device copy (torch), the calibration kernel for the
@@ -861,6 +868,17 @@ def main():
         line["generic"] = bench_generic()
     if not args.no_serve:
         line["serve"] = bench_serve(args.serve_seconds, args.serve_rate, args.serve_budget_gib)
+        # the same mix at higher offered rates: continuous batching under load,
+        # up to the rate the device and the HBM budget sustain
+        load = []
+        for rate in [float(r) for r in args.serve_load_rates.split(",") if r.strip()]:
+            r = bench_serve(args.serve_load_seconds, rate, args.serve_budget_gib)
+            load.append({k: r[k] for k in ("rate_offered", "seconds", "requests", "p50_ms", "p99_ms", "p999_ms",
+                                           "throughput_req_s", "payload_gbs", "all_ok", "bit_exact_sample",
+                                           "mean_requests_per_launch", "pool_peak_gib", "max_queue",
+                                           "stages_p50_ms")})
+        if load:
+            line["serve_load"] = load
     if not args.no_verify or not args.no_cpu:
         print("bench: CPU leg (checker and CPU baseline)", file=sys.stderr, flush=True)
         checks = {}
