@@ -378,7 +378,7 @@ launch_gtile_v(bool compress, const void* in, void* out, const gtile::gplan& g, 
 {
   const std::uint64_t tiles = (g.n + g.V - 1) / g.V;
   const std::size_t lds =
-    static_cast<std::size_t>(compress ? gtile::compress_lds_dwords(g) : gtile::image_dwords(g) + g.TB) * 4u;
+    static_cast<std::size_t>(compress ? gtile::compress_lds_dwords(g) : gtile::decompress_lds_dwords(g)) * 4u;
   const dim3 grid(static_cast<unsigned>(tiles));
   if (compress) {
     const int align = alignment_of(static_cast<const std::uint8_t*>(out) + g.R);

@@ -36,6 +36,9 @@ namespace frsz2 {
 namespace gtile {
 
 constexpr int WG = 256;
+#ifndef GT_RULE
+#define GT_RULE 2
+#endif
 constexpr int GT_MAX_VALUES = 4096;
 // tiles of up to TARGET_VALUES values take VPT = TARGET_VALUES / WG values
 // per thread, larger ones (TB rounded up for alignment) GT_MAX_VALUES / WG
@@ -125,6 +128,29 @@ move_payload(std::uint8_t* g, std::uint32_t* lds, std::uint32_t nbytes, int alig
       g[o] = lb[o];
     }
   }
+}
+
+// LDS of a decompress tile: the payload image, then (from a 16-byte aligned
+// dword offset) the tile's TB block exponents as they lie in the stream,
+// TB U/8 <= 2 TB dwords
+__host__ __device__ inline std::uint32_t
+exponent_offset(const gplan& p)
+{
+  return (image_dwords(p) + 3u) & ~3u;
+}
+
+__host__ __device__ inline std::uint32_t
+decompress_lds_dwords(const gplan& p)
+{
+  return exponent_offset(p) + 2u * p.TB;
+}
+
+// largest of 16, 8, 4 and 1 that divides the address
+__device__ __forceinline__ int
+alignment_at(const void* a)
+{
+  const std::uintptr_t v = reinterpret_cast<std::uintptr_t>(a);
+  return (v & 15u) == 0 ? 16 : (v & 7u) == 0 ? 8 : (v & 3u) == 0 ? 4 : 1;
 }
 
 // LDS dwords of a compress tile: payload image, TB block keys, V 16-bit value keys
@@ -248,8 +274,102 @@ compress_gtile(const T* __restrict__ in, std::uint8_t* __restrict__ out, const g
   }
 }
 
+// Decode of unpacked codes (bits = U <= 32) four consecutive values at a
+// time: chunk c of thread t holds tile values 4 (c WG + t) .. + 3, so one
+// LDS read of 4 U/8 bytes gives a chunk's codes and a wave's chunk stores
+// are one contiguous span (16-byte stores when the tile's output is 16-byte
+// aligned).  For bits = U a block's payload is its B codes with no padding
+// (CB = B U / 8), so code i is word i of the image, and V = TB B is a
+// multiple of 4 (TB CB is a multiple of 16 bytes), so every chunk that
+// starts below cnt reads inside the image.  Positions at or past cnt take
+// cnt - 1 for their exponent and are not stored.
+template<typename T, int U, int NC, bool NT>
+__device__ __forceinline__ void
+decode_chunks(const std::uint32_t* img, const std::uint8_t* exb, T* __restrict__ o, std::uint32_t cnt,
+              const gplan& p)
+{
+  static_assert(U <= 32, "chunks of four codes are read as at most 16 bytes");
+  using uw = typename fmt<T>::uw;
+  const std::uint32_t t = threadIdx.x;
+  const std::uint32_t last = cnt - 1;
+  uw code[NC][4];
+  int E[NC][4];
+  int emin = 0x7fffffff;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const std::uint32_t c0 = 4u * (t + static_cast<std::uint32_t>(c * WG));
+    const std::uint32_t i0 = c0 <= last ? c0 : (last & ~3u);
+    if constexpr (U == 8) {
+      const std::uint32_t w = img[i0 / 4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        code[c][q] = static_cast<uw>((w >> (8 * q)) & 0xFFu);
+      }
+    } else if constexpr (U == 16) {
+      const std::uint32_t w0 = img[i0 / 2];
+      const std::uint32_t w1 = img[i0 / 2 + 1];
+      code[c][0] = static_cast<uw>(w0 & 0xFFFFu);
+      code[c][1] = static_cast<uw>(w0 >> 16);
+      code[c][2] = static_cast<uw>(w1 & 0xFFFFu);
+      code[c][3] = static_cast<uw>(w1 >> 16);
+    } else {
+      const tile::u32x4 w = *reinterpret_cast<const tile::u32x4*>(img + i0);
+      code[c][0] = static_cast<uw>(w.x);
+      code[c][1] = static_cast<uw>(w.y);
+      code[c][2] = static_cast<uw>(w.z);
+      code[c][3] = static_cast<uw>(w.w);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const std::uint32_t i = i0 + static_cast<std::uint32_t>(q);
+      E[c][q] = load_block_exponent<U>(exb, block_of(i <= last ? i : last, p));
+      emin = E[c][q] < emin ? E[c][q] : emin;
+    }
+  }
+  // decode_value_short needs bits <= SB + 1 and E + bias >= 0 (every block a
+  // compress writes); checked over all of the thread's values
+  const bool fast = p.bits <= fmt<T>::SB + 1 && emin + fmt<T>::BIAS >= 0;
+  const bool vec = (reinterpret_cast<std::uintptr_t>(o) & 15u) == 0;
+  uw* ow = reinterpret_cast<uw*>(o);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const std::uint32_t c0 = 4u * (t + static_cast<std::uint32_t>(c * WG));
+    if (c0 > last) {
+      continue;
+    }
+    uw y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      y[q] = fast ? decode_value_short<T>(code[c][q], E[c][q], p.bits) : decode_value<T>(code[c][q], E[c][q], p.bits);
+    }
+    if (vec && c0 + 3 <= last) {
+      tile::u32x4* d = reinterpret_cast<tile::u32x4*>(ow + c0);
+      if constexpr (sizeof(T) == 4) {
+        const tile::u32x4 v = { static_cast<std::uint32_t>(y[0]), static_cast<std::uint32_t>(y[1]),
+                                static_cast<std::uint32_t>(y[2]), static_cast<std::uint32_t>(y[3]) };
+        tile::st<NT>(v, d);
+      } else {
+        const tile::u32x4 v0 = { static_cast<std::uint32_t>(y[0]), static_cast<std::uint32_t>(y[0] >> 32),
+                                 static_cast<std::uint32_t>(y[1]), static_cast<std::uint32_t>(y[1] >> 32) };
+        const tile::u32x4 v1 = { static_cast<std::uint32_t>(y[2]), static_cast<std::uint32_t>(y[2] >> 32),
+                                 static_cast<std::uint32_t>(y[3]), static_cast<std::uint32_t>(y[3] >> 32) };
+        tile::st<NT>(v0, d);
+        tile::st<NT>(v1, d + 1);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (c0 + static_cast<std::uint32_t>(q) <= last) {
+          tile::st<NT>(y[q], ow + c0 + q);
+        }
+      }
+    }
+  }
+}
+
 // Decompress tile `tile` of a run (exponents at in, payload at in + p.R)
-// into out.  LDS: image_dwords(p) dwords of payload, then TB exponents.
+// into out.  LDS: decompress_lds_dwords(p) (payload image, then the tile's
+// block exponents).
 // PACKED: bits < U; otherwise word i of the image is value i's code.
 template<typename T, int U, int VPT, bool PACKED, bool NT>
 __device__ __forceinline__ void
@@ -260,17 +380,32 @@ decompress_gtile(const std::uint8_t* __restrict__ in, T* __restrict__ out, const
   using wt = typename word<U>::type;
   const std::uint32_t t = threadIdx.x;
   std::uint32_t* img = smem;
-  int* ex = reinterpret_cast<int*>(smem + image_dwords(p));
+  const std::uint8_t* exb = reinterpret_cast<const std::uint8_t*>(smem + exponent_offset(p));
   const std::uint64_t k0 = tile * p.TB;
   const std::uint64_t base = k0 * p.B;
   const std::uint64_t left = p.n - base;
   const std::uint32_t cnt = left < p.V ? static_cast<std::uint32_t>(left) : p.V;
   const std::uint32_t nblk = block_of(cnt - 1, p) + 1;
-  for (std::uint32_t b = t; b < nblk; b += WG) {
-    ex[b] = load_block_exponent<U>(in, k0 + b);
+  // The tile's block exponents into LDS.  More blocks than threads (B < 9
+  // at the target tile size): the contiguous run of exponents moves with the
+  // widest loads its alignment allows, instead of several rounds of one load
+  // per block (float32 bits 8, B = 3: 4.2 -> 5.2 TB/s).  Fewer: one load per
+  // block and thread, which measured faster there (B = 12 and 100: 5-20%).
+  std::uint8_t* eg = const_cast<std::uint8_t*>(in) + k0 * static_cast<std::uint64_t>(U / 8);
+  std::uint8_t* el = reinterpret_cast<std::uint8_t*>(smem + exponent_offset(p));
+  if (nblk > static_cast<std::uint32_t>(WG)) {
+    move_payload<true, false>(eg, smem + exponent_offset(p), nblk * static_cast<std::uint32_t>(U / 8), alignment_at(eg));
+  } else {
+    for (std::uint32_t b = t; b < nblk; b += WG) {
+      store_block_exponent<U>(el, b, load_block_exponent<U>(in, k0 + b));
+    }
   }
   move_payload<true, NT>(const_cast<std::uint8_t*>(in) + p.R + k0 * p.CB, img, nblk * p.CB, align);
   __syncthreads();
+  if constexpr (!PACKED && U <= 32 && VPT % 4 == 0) {
+    decode_chunks<T, U, VPT / 4, NT>(img, exb, out + base, cnt, p);
+    return;
+  }
   const std::uint32_t bits = static_cast<std::uint32_t>(p.bits);
   const std::uint32_t padbits = p.CB * 8u - p.B * bits;
   uw code[VPT];
@@ -288,7 +423,7 @@ decompress_gtile(const std::uint8_t* __restrict__ in, T* __restrict__ out, const
     } else {
       code[r] = static_cast<uw>(reinterpret_cast<const wt*>(img)[i]);
     }
-    E[r] = ex[k];
+    E[r] = load_block_exponent<U>(exb, k);
     emin = E[r] < emin ? E[r] : emin;
   }
   // decode_value_short needs bits <= SB + 1 and E + bias >= 0 (every block a

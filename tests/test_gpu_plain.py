@@ -194,6 +194,30 @@ def test_stream_and_values_off_16_byte_alignment(fz, port, dtype, bits, B):
     assert np.array_equal(s.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("dtype,bits,B", [(np.float32, 8, 3), (np.float32, 16, 100), (np.float32, 32, 7),
+                                          (np.float64, 16, 5), (np.float64, 32, 12), (np.float32, 8, 1000)])
+def test_generic_unpacked_decode_into_any_alignment(fz, port, dtype, bits, B):
+    """Unpacked codes (bits = U) of block sizes that are not powers of two
+    decode four consecutive values per lane on the generic tiles: 16-byte
+    stores into aligned values, single stores into values one element off
+    16-byte alignment, and partial chunks at the end of the array."""
+    rng = np.random.default_rng(bits * 13 + B)
+    for n in (1, 3, 4095, 20011):
+        x = mixed_values(rng, n, dtype)
+        want = port.compress(x, bits, B)
+        ref = ubits(port.decompress(want, n, dtype, bits, B))
+        s = torch.from_numpy(want).cuda()
+        aligned = torch.full((n,), float("nan"), dtype=tdt(dtype), device="cuda")
+        fz.decompress_into(s, aligned, bits, B)
+        assert np.array_equal(ubits(aligned.cpu().numpy()), ref), (n, "aligned")
+        buf = torch.full((n + 1,), float("nan"), dtype=tdt(dtype), device="cuda")
+        off = buf[1:]
+        assert off.data_ptr() % 16 != 0
+        fz.decompress_into(s, off, bits, B)
+        assert np.array_equal(ubits(off.cpu().numpy()), ref), (n, "one element off")
+        assert np.isnan(buf[0].item())
+
+
 def test_gather_out_of_range_is_zero(fz):
     x = torch.ones(1000, dtype=torch.float32, device="cuda")
     s = fz.compress(x, 16, 32)
