@@ -15,6 +15,8 @@
 // contiguous range of tiles per XCD (workgroup b runs on XCD b mod 8), the
 // two choices the codec's tile kernels make.
 //
+// Vectors per lane per unit (UNR): 1, 2 or 4.
+//
 // Usage: stream_ceiling [--mib N]   (N MiB per side, default 1024)
 // Prints one JSON line per (pattern, workgroup size, UNR, NT, order) with
 // the best of three rounds of ten launches, event-timed, and exits non-zero
@@ -270,24 +272,34 @@ main(int argc, char** argv)
   for (int wg : {256, 64})
     for (int xcd : {0, 1}) {
       CEIL_RUN("read-only", 1, 0, 4, false);
+      CEIL_RUN("read-only", 1, 0, 2, false);
       CEIL_RUN("read-only", 1, 0, 1, false);
       CEIL_RUN("read-only", 1, 0, 4, true);
+      CEIL_RUN("read-only", 1, 0, 2, true);
       CEIL_RUN("read-only", 1, 0, 1, true);
       CEIL_RUN("write-only", 0, 1, 4, false);
+      CEIL_RUN("write-only", 0, 1, 2, false);
       CEIL_RUN("write-only", 0, 1, 1, false);
       CEIL_RUN("write-only", 0, 1, 4, true);
+      CEIL_RUN("write-only", 0, 1, 2, true);
       CEIL_RUN("write-only", 0, 1, 1, true);
       CEIL_RUN("copy 1:1", 1, 1, 4, false);
+      CEIL_RUN("copy 1:1", 1, 1, 2, false);
       CEIL_RUN("copy 1:1", 1, 1, 1, false);
       CEIL_RUN("copy 1:1", 1, 1, 4, true);
+      CEIL_RUN("copy 1:1", 1, 1, 2, true);
       CEIL_RUN("copy 1:1", 1, 1, 1, true);
       CEIL_RUN("2:1", 2, 1, 4, false);
+      CEIL_RUN("2:1", 2, 1, 2, false);
       CEIL_RUN("2:1", 2, 1, 1, false);
       CEIL_RUN("2:1", 2, 1, 4, true);
+      CEIL_RUN("2:1", 2, 1, 2, true);
       CEIL_RUN("2:1", 2, 1, 1, true);
       CEIL_RUN("1:2", 1, 2, 4, false);
+      CEIL_RUN("1:2", 1, 2, 2, false);
       CEIL_RUN("1:2", 1, 2, 1, false);
       CEIL_RUN("1:2", 1, 2, 4, true);
+      CEIL_RUN("1:2", 1, 2, 2, true);
       CEIL_RUN("1:2", 1, 2, 1, true);
     }
 #undef CEIL_RUN

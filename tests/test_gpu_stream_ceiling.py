@@ -23,4 +23,4 @@ def test_stream_ceiling_patterns():
     rows = [json.loads(ln) for ln in res.stdout.splitlines() if ln.startswith("{")]
     patterns = {(r["read"], r["write"]) for r in rows}
     assert patterns == {(1, 0), (0, 1), (1, 1), (2, 1), (1, 2)}
-    assert len(rows) == 80 and all(r["gbs"] > 0 for r in rows)
+    assert len(rows) == 120 and all(r["gbs"] > 0 for r in rows)
