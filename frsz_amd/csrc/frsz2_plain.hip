@@ -83,19 +83,18 @@ __global__ __launch_bounds__(WG) void k_compress_block(const T* __restrict__ in,
                                                        layout L,
                                                        std::uint32_t zpad)
 {
-  const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
-  if (gid < 256) {
+  const std::uint64_t gid0 = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (gid0 < 256) {
     std::uint8_t* q = out + L.nb * static_cast<std::uint64_t>(U / 8);
-    for (std::uint32_t i = static_cast<std::uint32_t>(gid); i < zpad; i += 256) {
+    for (std::uint32_t i = static_cast<std::uint32_t>(gid0); i < zpad; i += 256) {
       q[i] = 0;
     }
   }
-  if (gid >= L.nb) {
-    return;
+  for (std::uint64_t gid = gid0; gid < L.nb; gid += static_cast<std::uint64_t>(gridDim.x) * WG) {
+    const std::uint64_t first = gid * L.B;
+    const std::uint64_t len = (L.n - first) < L.B ? (L.n - first) : L.B;
+    encode_block<T, U>(out, L, gid, in + first, len);
   }
-  const std::uint64_t first = gid * L.B;
-  const std::uint64_t len = (L.n - first) < L.B ? (L.n - first) : L.B;
-  encode_block<T, U>(out, L, gid, in + first, len);
 }
 
 // Generic compress, larger blocks (any B, any alignment), in two passes that
@@ -115,32 +114,38 @@ __global__ __launch_bounds__(WG) void k_block_exponents(const T* __restrict__ in
                                                         int lg)
 {
   using uw = typename fmt<T>::uw;
-  const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
-  if (gid < 256) {
+  const std::uint64_t gid0 = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
+  if (gid0 < 256) {
     std::uint8_t* q = out + L.nb * static_cast<std::uint64_t>(U / 8);
-    for (std::uint32_t i = static_cast<std::uint32_t>(gid); i < zpad; i += 256) {
+    for (std::uint32_t i = static_cast<std::uint32_t>(gid0); i < zpad; i += 256) {
       q[i] = 0;
     }
   }
   const int G = 1 << lg;
-  const std::uint64_t k = gid >> lg;
-  const std::uint64_t lane = gid & static_cast<std::uint64_t>(G - 1);
-  std::uint32_t m = 0;
-  if (k < L.nb) {
-    const std::uint64_t first = k * L.B;
-    const std::uint64_t count = (L.n - first) < L.B ? (L.n - first) : L.B;
-    for (std::uint64_t j = lane; j < count; j += static_cast<std::uint64_t>(G)) {
-      const std::uint32_t key = max_key<T>(__builtin_bit_cast(uw, in[first + j]));
-      m = m > key ? m : key;
+  const std::uint64_t total = L.nb << lg;
+  // grid-stride over workgroup-sized steps: the loop condition is uniform
+  // across the workgroup, so every lane of a wave takes part in each exchange
+  for (std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * WG; base < total;
+       base += static_cast<std::uint64_t>(gridDim.x) * WG) {
+    const std::uint64_t gid = base + threadIdx.x;
+    const std::uint64_t k = gid >> lg;
+    const std::uint64_t lane = gid & static_cast<std::uint64_t>(G - 1);
+    std::uint32_t m = 0;
+    if (k < L.nb) {
+      const std::uint64_t first = k * L.B;
+      const std::uint64_t count = (L.n - first) < L.B ? (L.n - first) : L.B;
+      for (std::uint64_t j = lane; j < count; j += static_cast<std::uint64_t>(G)) {
+        const std::uint32_t key = max_key<T>(__builtin_bit_cast(uw, in[first + j]));
+        m = m > key ? m : key;
+      }
     }
-  }
-  // every lane of the wave takes part in the exchange (none has returned)
-  for (int o = G >> 1; o > 0; o >>= 1) {
-    const std::uint32_t other = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(m), o, G));
-    m = m > other ? m : other;
-  }
-  if (k < L.nb && lane == 0) {
-    store_block_exponent<U>(out, k, block_exponent_from_key<T, U>(m));
+    for (int o = G >> 1; o > 0; o >>= 1) {
+      const std::uint32_t other = static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(m), o, G));
+      m = m > other ? m : other;
+    }
+    if (k < L.nb && lane == 0) {
+      store_block_exponent<U>(out, k, block_exponent_from_key<T, U>(m));
+    }
   }
 }
 
@@ -150,19 +155,14 @@ __global__ __launch_bounds__(WG) void k_block_exponents(const T* __restrict__ in
 // a block fit 32 bits (B <= 2^24, bits <= 64), and so does the word index
 // when the stream has fewer than 2^32 payload words (narrow = true), which
 // keeps the divisions 32-bit.
+// payload word gid (of block gid / wpb) of the generic two-pass compress
 template<typename T, int U>
-__global__ __launch_bounds__(WG) void k_encode_words(const T* __restrict__ in,
-                                                     std::uint8_t* __restrict__ out,
-                                                     layout L,
-                                                     std::uint32_t wpb,
-                                                     bool narrow)
+__device__ __forceinline__ void
+encode_word(const T* __restrict__ in, std::uint8_t* __restrict__ out, const layout& L, std::uint32_t wpb, bool narrow,
+            std::uint64_t gid)
 {
   using uw = typename fmt<T>::uw;
   using wt = typename word<U>::type;
-  const std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
-  if (gid >= L.nb * wpb) {
-    return;
-  }
   std::uint64_t k;
   std::uint32_t w;
   if (narrow) {
@@ -198,6 +198,20 @@ __global__ __launch_bounds__(WG) void k_encode_words(const T* __restrict__ in,
   reinterpret_cast<wt*>(out + L.R + k * L.CB)[w] = static_cast<wt>(acc);
 }
 
+template<typename T, int U>
+__global__ __launch_bounds__(WG) void k_encode_words(const T* __restrict__ in,
+                                                     std::uint8_t* __restrict__ out,
+                                                     layout L,
+                                                     std::uint32_t wpb,
+                                                     bool narrow)
+{
+  const std::uint64_t words = L.nb * wpb;
+  for (std::uint64_t gid = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x; gid < words;
+       gid += static_cast<std::uint64_t>(gridDim.x) * WG) {
+    encode_word<T, U>(in, out, L, wpb, narrow, gid);
+  }
+}
+
 // Generic-B tiles (gtile.hpp): one workgroup per tile of TB whole blocks;
 // `align` is the alignment of the payload (16, 8, 4 or 1 bytes)
 template<typename T, int U, int VPT, bool PACKED>
@@ -225,8 +239,8 @@ __global__ __launch_bounds__(WG) void k_decompress_elem(const std::uint8_t* __re
                                                         T* __restrict__ out,
                                                         layout L)
 {
-  const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
-  if (i < L.n) {
+  for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x; i < L.n;
+       i += static_cast<std::uint64_t>(gridDim.x) * WG) {
     out[i] = decode_element<T, U>(in, L, i);
   }
 }
@@ -238,8 +252,8 @@ __global__ __launch_bounds__(WG) void k_gather(const std::uint8_t* __restrict__ 
                                                std::uint64_t m,
                                                T* __restrict__ out)
 {
-  const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x;
-  if (i < m) {
+  for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * WG + threadIdx.x; i < m;
+       i += static_cast<std::uint64_t>(gridDim.x) * WG) {
     const std::uint64_t j = idx[i];
     out[i] = j < L.n ? decode_element<T, U, true>(in, L, j) : T(0);
   }
@@ -265,7 +279,11 @@ ilog2_exact(std::uint64_t B)
 inline unsigned
 grid_for(std::uint64_t work)
 {
-  return static_cast<unsigned>((work + WG - 1) / WG);
+  // the one-thread-per-item kernels stride over the grid: at most 2^22
+  // workgroups (2^30 work-items, within the 32-bit work-item count of a
+  // dispatch whatever the array's length)
+  const std::uint64_t g = (work + WG - 1) / WG;
+  return static_cast<unsigned>(g < (std::uint64_t{ 1 } << 22) ? g : (std::uint64_t{ 1 } << 22));
 }
 
 template<typename T, int U, int VPT, bool NT, int WGS, bool VEC = true>

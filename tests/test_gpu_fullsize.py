@@ -68,13 +68,15 @@ def test_field_1gib_full_size(port_mt):
     assert bool((err < torch.exp2((E - 30).double())).all())
 
 
-@pytest.mark.parametrize("bits,B", [(12, 32), (32, 64)])
+@pytest.mark.parametrize("bits,B", [(12, 32), (32, 64), (8, 1), (12, 5000)])
 def test_more_than_2p32_values(port, bits, B):
     """Maximum sizes: n = 2^32 + 4133 float32 values (17 GB in, value and byte
     offsets past 32 bits).  Blocks are independent, so block k of the stream
     equals the one-block stream of its values: checked by the oracle for the
     first blocks, those around value 2^32 and the partial last block, with the
-    exponent pad, the decoded values there and gathers past 2^32."""
+    exponent pad, the decoded values there and gathers past 2^32.  B = 1 and
+    B = 5000 take the one-thread-per-block / per-word / per-value kernels,
+    whose grids then stride over more than 2^32 items."""
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     import frsz_amd
@@ -110,3 +112,30 @@ def test_more_than_2p32_values(port, bits, B):
     yv = y[idx[:5]].cpu().numpy().view(np.uint32)
     assert np.array_equal(gv[:5], yv)
     assert np.array_equal(gv[5:], np.zeros(2, np.uint32)), "indices >= n gather 0"
+
+
+def test_gather_more_than_2p32_indices():
+    """A gather of 2^32 + 7 indices in one call (34 GB of indices): the
+    one-thread-per-index kernel strides over more work-items than one
+    dispatch holds; every output (and, printed on failure, samples around
+    index 2^32 and at the end) equals the decoded value it indexes."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frsz_amd
+
+    n, bits, B = 1 << 20, 16, 32
+    m = (1 << 32) + 7
+    print(f"gather of {m} indices", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    x = torch.randn(n, device="cuda", generator=g)
+    s = frsz_amd.compress(x, bits, B)
+    y = frsz_amd.decompress(s, n, torch.float32, bits, B)
+    idx = torch.randint(0, n, (m,), device="cuda", dtype=torch.int64, generator=g)
+    out = frsz_amd.gather(s, n, torch.float32, bits, B, idx)
+    pos = torch.tensor([0, 1, 12345, (1 << 32) - 2, (1 << 32) - 1, 1 << 32, m - 2, m - 1], device="cuda")
+    got = out[pos].cpu().numpy().view(np.uint32)
+    want = y[idx[pos]].cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, want)
+    # every output: bit patterns equal to the decoded values they index
+    assert torch.equal(out.view(torch.int32), y[idx].view(torch.int32))
