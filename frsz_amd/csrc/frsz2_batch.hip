@@ -235,6 +235,12 @@ __global__ __launch_bounds__(WGS) void k_grouped_inline(job_pack pack,
   grouped_tile<T, U, VPT, PACKED, NT, STAGE, OP, WGS>(pack.j, njobs, tile_base, ntiles, xcd, nullptr);
 }
 
+// Tiles per launch of one workgroup per tile: 2^22 tiles of at most 256
+// threads stay below the 2^32 work-items of one dispatch (the classes of the
+// stated 4096-array workload hold 3.8-8.4 million tiles, so most of them
+// take two launches).
+constexpr std::uint64_t MAX_LAUNCH_TILES = std::uint64_t{ 1 } << 22;
+
 // The grouped kernel of class CLS over its descriptors d[0..nd), tiles
 // [t0, t0 + nt).  h: host copy of the descriptors when they go in the kernel
 // arguments (nd <= INLINE_JOBS, one workgroup per tile), else null.
@@ -267,14 +273,23 @@ launch_class(const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64
       grid = grid / 8 * 8;
     }
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(WGS), 0, s, d, nd, t0, nt, xcd, jtab);
-  } else if (h != nullptr) {
-    job_pack pack{};
-    std::copy(h, h + nd, pack.j);
-    hipLaunchKernelGGL((k_grouped_inline<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)),
-                       dim3(WGS), 0, s, pack, nd, t0, nt, xcd);
   } else {
-    hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(nt)),
-                       dim3(WGS), 0, s, d, nd, t0, nt, xcd, jtab);
+    // one workgroup per tile, at most MAX_LAUNCH_TILES tiles per launch (a
+    // dispatch counts its work-items in 32 bits)
+    job_pack pack{};
+    if (h != nullptr) {
+      std::copy(h, h + nd, pack.j);
+    }
+    for (std::uint64_t c0 = 0; c0 < nt; c0 += MAX_LAUNCH_TILES) {
+      const std::uint64_t cn = nt - c0 < MAX_LAUNCH_TILES ? nt - c0 : MAX_LAUNCH_TILES;
+      if (h != nullptr) {
+        hipLaunchKernelGGL((k_grouped_inline<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(cn)),
+                           dim3(WGS), 0, s, pack, nd, t0 + c0, cn, xcd);
+      } else {
+        hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(cn)),
+                           dim3(WGS), 0, s, d, nd, t0 + c0, cn, xcd, jtab);
+      }
+    }
   }
 }
 

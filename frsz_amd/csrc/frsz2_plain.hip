@@ -794,6 +794,17 @@ decompress_run(int dtype, const void* d_in, void* d_values, const layout& L, hip
                     "decompress launch");
 }
 
+// Blocks per launch of a long run: a whole multiple of 16 blocks (each
+// launch's payload then keeps the 16-byte alignment of the whole run's, and
+// its values the alignment of the run's values) holding at most 2^30 values,
+// so no launch comes near the 2^32 work-items a dispatch can count.
+std::uint64_t
+launch_blocks(std::uint64_t block)
+{
+  const std::uint64_t b = (std::uint64_t{ 1 } << 30) / block / 16 * 16;
+  return b > 16 ? b : 16;
+}
+
 int
 compress_blocks_device(int dtype, int bits, std::uint64_t block, std::uint64_t n, std::uint64_t first,
                        const void* d_values, std::uint64_t count, void* d_stream, std::uint64_t stream_bytes,
@@ -812,8 +823,21 @@ compress_blocks_device(int dtype, int bits, std::uint64_t block, std::uint64_t n
   if (!d_stream || (count > 0 && !d_values) || !aligned(d_values, vs) || !aligned(d_stream, 8)) {
     return set_error(FRSZ2_EARG, "null or misaligned pointer");
   }
-  const run_plan r = make_run(bits, block, n, first, count);
-  return compress_run(dtype, d_values, static_cast<std::uint8_t*>(d_stream) + r.exp_off, r.L, r.zpad, s);
+  const std::uint64_t step = launch_blocks(block);
+  std::uint64_t k = first;
+  std::uint64_t left = count;
+  do {
+    const std::uint64_t take = left < step * block ? left : step * block;
+    const run_plan r = make_run(bits, block, n, k, take);
+    rc = compress_run(dtype, static_cast<const std::uint8_t*>(d_values) + (k - first) * block * vs,
+                      static_cast<std::uint8_t*>(d_stream) + r.exp_off, r.L, r.zpad, s);
+    if (rc) {
+      return rc;
+    }
+    k += step;
+    left -= take;
+  } while (left > 0);
+  return FRSZ2_OK;
 }
 
 int
@@ -834,8 +858,21 @@ decompress_blocks_device(int dtype, int bits, std::uint64_t block, std::uint64_t
   if (!d_stream || (count > 0 && !d_values) || !aligned(d_stream, 8) || !aligned(d_values, vs)) {
     return set_error(FRSZ2_EARG, "null or misaligned pointer");
   }
-  const run_plan r = make_run(bits, block, n, first, count);
-  return decompress_run(dtype, static_cast<const std::uint8_t*>(d_stream) + r.exp_off, d_values, r.L, s);
+  const std::uint64_t step = launch_blocks(block);
+  std::uint64_t k = first;
+  std::uint64_t left = count;
+  do {
+    const std::uint64_t take = left < step * block ? left : step * block;
+    const run_plan r = make_run(bits, block, n, k, take);
+    rc = decompress_run(dtype, static_cast<const std::uint8_t*>(d_stream) + r.exp_off,
+                        static_cast<std::uint8_t*>(d_values) + (k - first) * block * vs, r.L, s);
+    if (rc) {
+      return rc;
+    }
+    k += step;
+    left -= take;
+  } while (left > 0);
+  return FRSZ2_OK;
 }
 
 int
