@@ -2,7 +2,8 @@
 
     jobs = [batch.compress_job(x, stream_buf, bits=16, block=32), ...]
     plan = batch.BatchPlan(jobs)          # descriptors uploaded, launch captured in a hipGraph
-    plan.autotune()                        # optional: time the tile schedules on these jobs
+    plan.autotune()                        # optional: time the tile schedules and each class's
+                                           # tile shape on these jobs
     plan.launch()                          # one graph launch on the current stream
 
 Each job means exactly one frsz_amd.compress_into / decompress_into call
@@ -45,6 +46,20 @@ SCHEDULE_TILES = 0       # one workgroup per tile
 SCHEDULE_PERSISTENT = 1  # per class, a grid of what the device holds striding over its tiles
 
 
+BATCH_CLASSES = 28  # kernel classes of tiled compress / decompress jobs (FRSZ2_BATCH_CLASSES)
+
+
+def job_class(op: int, dtype: torch.dtype, bits: int) -> int:
+    """Kernel class of a compress or decompress job (op * 14 + slot; slot
+    2u + packed for float32, 6 + 2u + packed for float64, with u the index
+    of the storage width U in (8, 16, 32, 64) and packed = bits < U)."""
+    U = 8 if bits <= 8 else 16 if bits <= 16 else 32 if bits <= 32 else 64
+    u = (8, 16, 32, 64).index(U)
+    packed = 1 if bits < U else 0
+    slot = 2 * u + packed if dtype == torch.float32 else 6 + 2 * u + packed
+    return op * 14 + slot
+
+
 class BatchConfig(ctypes.Structure):
     _fields_ = [
         ("schedule", ctypes.c_int),
@@ -52,15 +67,27 @@ class BatchConfig(ctypes.Structure):
         ("xcd_order", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("plain_launches", ctypes.c_int),
+        ("values_per_thread", ctypes.c_ubyte * BATCH_CLASSES),
+        ("workgroup_size", ctypes.c_ushort * BATCH_CLASSES),
     ]
 
     @classmethod
     def from_dict(cls, d: dict) -> "BatchConfig":
-        return cls(d.get("schedule", SCHEDULE_TILES), d.get("workgroups", 0), d.get("xcd_order", 0),
-                   d.get("use_graph", 1), d.get("plain_launches", 0))
+        """``d["tile_shapes"]`` maps a kernel class (job_class) to
+        ``[values_per_thread, workgroup_size]`` (0 = the built-in value);
+        classes not named run their built-in shape."""
+        c = cls(d.get("schedule", SCHEDULE_TILES), d.get("workgroups", 0), d.get("xcd_order", 0),
+                d.get("use_graph", 1), d.get("plain_launches", 0))
+        for k, (vpt, wgs) in d.get("tile_shapes", {}).items():
+            c.values_per_thread[int(k)] = vpt
+            c.workgroup_size[int(k)] = wgs
+        return c
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_[:5]}
+        d["tile_shapes"] = {c: [self.values_per_thread[c], self.workgroup_size[c]] for c in range(BATCH_CLASSES)
+                            if self.values_per_thread[c] or self.workgroup_size[c]}
+        return d
 
 
 _L = _capi.lib
@@ -178,6 +205,6 @@ class BatchPlan:
             pass
 
 
-__all__ = ["BatchPlan", "BatchConfig", "run_noalloc", "NOALLOC_MAX_JOBS", "compress_job", "decompress_job",
+__all__ = ["BatchPlan", "BatchConfig", "BATCH_CLASSES", "job_class", "run_noalloc", "NOALLOC_MAX_JOBS", "compress_job", "decompress_job",
            "gather_job", "compressed_size", "OP_COMPRESS", "OP_DECOMPRESS", "OP_GATHER", "SCHEDULE_TILES",
            "SCHEDULE_PERSISTENT"]

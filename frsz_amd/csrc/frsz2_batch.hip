@@ -241,14 +241,80 @@ __global__ __launch_bounds__(WGS) void k_grouped_inline(job_pack pack,
 // take two launches).
 constexpr std::uint64_t MAX_LAUNCH_TILES = std::uint64_t{ 1 } << 22;
 
+// One workgroup per tile of class CLS in the tile shape VPT x WGS (the
+// class's non-temporal hint and staged stores), descriptors d[0..nd) in
+// device memory, tiles [t0, t0 + nt), at most MAX_LAUNCH_TILES tiles per
+// launch (a dispatch counts its work-items in 32 bits).
+template<int CLS, int VPT, int WGS>
+void
+launch_tiles(const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt, const std::uint32_t* jtab, hipStream_t s,
+             int xcd)
+{
+  using T = std::conditional_t<cls_dtype(CLS) == 0, float, double>;
+  constexpr int U = 8 << cls_ui(CLS);
+  constexpr int V = cls_default(CLS);
+  constexpr int OP = cls_op(CLS);
+  constexpr bool PACKED = cls_packed(CLS) != 0;
+  constexpr bool NT = venc_nt(V) != 0;
+  constexpr bool STAGE = OP == 1 && venc_stage(V) != 0;
+  for (std::uint64_t c0 = 0; c0 < nt; c0 += MAX_LAUNCH_TILES) {
+    const std::uint64_t cn = nt - c0 < MAX_LAUNCH_TILES ? nt - c0 : MAX_LAUNCH_TILES;
+    hipLaunchKernelGGL((k_grouped<T, U, VPT, PACKED, NT, STAGE, OP, WGS>), dim3(static_cast<unsigned>(cn)), dim3(WGS),
+                       0, s, d, nd, t0 + c0, cn, xcd, jtab);
+  }
+}
+
+// the same in the tile shape vpt x wgs chosen at run time (a shape
+// validate_shapes accepts for the class)
+template<int CLS>
+void
+launch_tiles_shape(int vpt, int wgs, const bjob* d, int nd, std::uint64_t t0, std::uint64_t nt,
+                   const std::uint32_t* jtab, hipStream_t s, int xcd)
+{
+  if constexpr (cls_op(CLS) != 2) {
+    constexpr bool two = cls_dtype(CLS) == 1 && cls_ui(CLS) >= 1; // 2 values per thread: one 16-byte float64 chunk
+#define FRSZ2_TS(V, W) launch_tiles<CLS, V, W>(d, nd, t0, nt, jtab, s, xcd)
+    if (wgs == 64) {
+      switch (vpt) {
+        case 2:
+          if constexpr (two) {
+            FRSZ2_TS(2, 64);
+          }
+          break;
+        case 4: FRSZ2_TS(4, 64); break;
+        case 8: FRSZ2_TS(8, 64); break;
+        default: FRSZ2_TS(16, 64); break;
+      }
+    } else {
+      switch (vpt) {
+        case 2:
+          if constexpr (two) {
+            FRSZ2_TS(2, WG);
+          }
+          break;
+        case 4: FRSZ2_TS(4, WG); break;
+        case 8: FRSZ2_TS(8, WG); break;
+        default: FRSZ2_TS(16, WG); break;
+      }
+    }
+#undef FRSZ2_TS
+  }
+}
+
 // The grouped kernel of class CLS over its descriptors d[0..nd), tiles
 // [t0, t0 + nt).  h: host copy of the descriptors when they go in the kernel
-// arguments (nd <= INLINE_JOBS, one workgroup per tile), else null.
+// arguments (nd <= INLINE_JOBS, one workgroup per tile, built-in shape),
+// else null.  vpt x wgs: the tile shape of the one-workgroup-per-tile
+// schedule (the persistent schedule runs the built-in shape).
 template<int CLS>
 void
 launch_class(const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64_t nt, const std::uint32_t* jtab,
-             hipStream_t s, const batch_schedule& sc)
+             hipStream_t s, const batch_schedule& sc, int vpt, int wgs)
 {
+  if (!sc.persistent && (vpt != class_vpt(CLS) || wgs != class_wgs(CLS))) {
+    launch_tiles_shape<CLS>(vpt, wgs, d, nd, t0, nt, jtab, s, sc.xcd_order ? 1 : 0);
+    return;
+  }
   using T = std::conditional_t<cls_dtype(CLS) == 0, float, double>;
   constexpr int U = 8 << cls_ui(CLS);
   constexpr int V = cls_default(CLS);
@@ -313,14 +379,18 @@ aligned16(const void* p)
 }
 
 // the grouped kernel of class cls over descriptors d[0..nd) (h: their host
-// copy when they go in the kernel arguments), tiles [t0, t0 + nt)
+// copy when they go in the kernel arguments), tiles [t0, t0 + nt), in the
+// class's tile shape under sh
 hipError_t
 launch_class_of(int cls, const bjob* d, const bjob* h, int nd, std::uint64_t t0, std::uint64_t nt,
-                const std::uint32_t* jtab, hipStream_t s, const batch_schedule& sc)
+                const std::uint32_t* jtab, hipStream_t s, const batch_schedule& sc, const class_shapes* sh = nullptr)
 {
+  int vpt = 0;
+  int wgs = 0;
+  class_shape(cls, sh, vpt, wgs);
 #define FRSZ2_G(K)                                                                                           \
   case K:                                                                                                    \
-    launch_class<K>(d, h, nd, t0, nt, jtab, s, sc);                                                          \
+    launch_class<K>(d, h, nd, t0, nt, jtab, s, sc, vpt, wgs);                                                \
     break
   switch (cls) {
     FRSZ2_G(0); FRSZ2_G(1); FRSZ2_G(2); FRSZ2_G(3); FRSZ2_G(4); FRSZ2_G(5); FRSZ2_G(6);
@@ -339,7 +409,7 @@ launch_class_of(int cls, const bjob* d, const bjob* h, int nd, std::uint64_t t0,
 // built in class order)
 hipError_t
 launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t total_tiles, const std::uint32_t* jtab,
-               bool inl, hipStream_t s, const batch_schedule& sc)
+               bool inl, hipStream_t s, const batch_schedule& sc, const class_shapes* sh)
 {
   int d0 = 0;
   while (d0 < nd) {
@@ -352,7 +422,7 @@ launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t tot
     const std::uint64_t t1 = d1 < nd ? h_desc[d1].tile0 : total_tiles;
     if (t1 > t0) {
       const hipError_t e =
-        launch_class_of(cls, d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s, sc);
+        launch_class_of(cls, d_desc + d0, inl ? h_desc + d0 : nullptr, d1 - d0, t0, t1 - t0, jtab, s, sc, sh);
       if (e != hipSuccess) {
         return e;
       }
@@ -367,6 +437,68 @@ launch_grouped(const bjob* d_desc, const bjob* h_desc, int nd, std::uint64_t tot
 } // namespace frsz2
 
 namespace frsz2 {
+
+int
+job_class(const frsz2_job& j)
+{
+  return class_of(j.op, j.dtype, j.bits);
+}
+
+void
+class_shape(int cls, const class_shapes* sh, int& vpt, int& wgs)
+{
+  vpt = class_vpt(cls);
+  wgs = class_wgs(cls);
+  if (sh != nullptr && cls >= 0 && cls < FRSZ2_BATCH_CLASSES) {
+    if (sh->vpt[cls] != 0) {
+      vpt = sh->vpt[cls];
+    }
+    if (sh->wgs[cls] != 0) {
+      wgs = sh->wgs[cls];
+    }
+  }
+}
+
+int
+validate_shapes(const class_shapes& sh)
+{
+  for (int c = 0; c < FRSZ2_BATCH_CLASSES; ++c) {
+    const int v = sh.vpt[c];
+    const int w = sh.wgs[c];
+    const bool two = cls_dtype(c) == 1 && cls_ui(c) >= 1;
+    if (!(v == 0 || v == 4 || v == 8 || v == 16 || (v == 2 && two))) {
+      return set_error(FRSZ2_EARG, "class " + std::to_string(c) + ": values per thread " + std::to_string(v) +
+                                     (two ? " (0, 2, 4, 8 or 16)" : " (0, 4, 8 or 16)"));
+    }
+    if (!(w == 0 || w == 64 || w == 256)) {
+      return set_error(FRSZ2_EARG,
+                       "class " + std::to_string(c) + ": workgroup size " + std::to_string(w) + " (0, 64 or 256)");
+    }
+  }
+  return FRSZ2_OK;
+}
+
+namespace {
+
+// true when every class runs its built-in shape under sh
+bool
+builtin_shapes(const class_shapes* sh)
+{
+  if (sh == nullptr) {
+    return true;
+  }
+  for (int c = 0; c < FRSZ2_BATCH_CLASSES; ++c) {
+    int v = 0;
+    int w = 0;
+    class_shape(c, sh, v, w);
+    if (v != class_vpt(c) || w != class_wgs(c)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+} // namespace
 
 int
 validate_job(const frsz2_job& j, int idx)
@@ -409,7 +541,7 @@ validate_job(const frsz2_job& j, int idx)
 // not a power of two or larger than the class's tile, pointers off 16-byte
 // alignment), in which case it runs as a plain launch.
 std::int64_t
-describe_job(const frsz2_job& j, std::uint64_t tile0, bjob* d)
+describe_job(const frsz2_job& j, std::uint64_t tile0, bjob* d, const class_shapes* sh = nullptr)
 {
   const int cls = class_of(j.op, j.dtype, j.bits);
   const layout L = layout::make(j.bits, j.block, j.n);
@@ -418,7 +550,10 @@ describe_job(const frsz2_job& j, std::uint64_t tile0, bjob* d)
   if (j.op == FRSZ2_OP_GATHER) {
     nt = (j.m + GATHER_TILE - 1) / GATHER_TILE;
   } else {
-    const std::uint64_t TILE = static_cast<std::uint64_t>(class_wgs(cls)) * class_vpt(cls);
+    int vpt = 0;
+    int wgs = 0;
+    class_shape(cls, sh, vpt, wgs);
+    const std::uint64_t TILE = static_cast<std::uint64_t>(wgs) * static_cast<std::uint64_t>(vpt);
     if (lb < 0 || j.block > TILE || !aligned16(j.in) || !aligned16(j.out)) {
       return -1;
     }
@@ -445,7 +580,8 @@ describe_job(const frsz2_job& j, std::uint64_t tile0, bjob* d)
 }
 
 std::uint64_t
-build_descriptors(const frsz2_job* jobs, std::size_t n, std::vector<bjob>& desc, std::vector<char>& tiled)
+build_descriptors(const frsz2_job* jobs, std::size_t n, std::vector<bjob>& desc, std::vector<char>& tiled,
+                  const class_shapes* sh)
 {
   tiled.assign(n, 0);
   std::uint64_t tiles = 0;
@@ -460,7 +596,7 @@ build_descriptors(const frsz2_job* jobs, std::size_t n, std::vector<bjob>& desc,
   });
   for (const std::size_t i : order) {
     bjob d;
-    const std::int64_t nt = describe_job(jobs[i], tiles, &d);
+    const std::int64_t nt = describe_job(jobs[i], tiles, &d, sh);
     if (nt < 0) {
       continue;
     }
@@ -493,21 +629,22 @@ tile_table(const std::vector<bjob>& desc, std::uint64_t total_tiles, bool within
 }
 
 bool
-descriptors_inline(const batch_schedule& sc, int ndesc, std::uint64_t total_tiles)
+descriptors_inline(const batch_schedule& sc, int ndesc, std::uint64_t total_tiles, const class_shapes* sh)
 {
-  // the persistent kernels read device descriptors
-  return ndesc > 0 && ndesc <= INLINE_JOBS && total_tiles > 0 && !sc.persistent;
+  // the persistent kernels read device descriptors; the inline kernels exist
+  // in the built-in shapes only
+  return ndesc > 0 && ndesc <= INLINE_JOBS && total_tiles > 0 && !sc.persistent && builtin_shapes(sh);
 }
 
 hipError_t
 launch_batch(const bjob* d_desc, const bjob* h_desc, int ndesc, std::uint64_t total_tiles, const batch_schedule& sc,
-             hipStream_t s, const std::uint32_t* d_jtab)
+             hipStream_t s, const std::uint32_t* d_jtab, const class_shapes* sh)
 {
   if (ndesc == 0 || total_tiles == 0) {
     return hipSuccess;
   }
   return launch_grouped(d_desc, h_desc, ndesc, total_tiles, d_jtab,
-                        h_desc != nullptr && descriptors_inline(sc, ndesc, total_tiles), s, sc);
+                        h_desc != nullptr && descriptors_inline(sc, ndesc, total_tiles, sh), s, sc, sh);
 }
 
 int
@@ -546,6 +683,8 @@ struct frsz2_batch_plan
   std::uint32_t* d_jtab = nullptr; // tile -> descriptor index within its class
   int ndesc = 0;
   std::uint64_t total_tiles = 0;
+  bool uploaded = false;       // descriptors built for `shapes`
+  frsz2::class_shapes shapes{}; // the tile shapes the descriptors were built with
   frsz2_batch_config cfg{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -554,36 +693,111 @@ struct frsz2_batch_plan
 namespace frsz2 {
 namespace {
 
-// build and upload the descriptors and the tile table (once per plan)
-int
-upload_descriptors(frsz2_batch_plan* P)
+// Device copies of a descriptor set and its tile table (within-class
+// indices), built on the host with shapes sh.
+struct device_descriptors
 {
-  P->host_desc.clear();
-  P->total_tiles = build_descriptors(P->jobs.data(), P->jobs.size(), P->host_desc, P->tiled);
-  P->ndesc = static_cast<int>(P->host_desc.size());
-  if (P->ndesc > 0) {
-    hipError_t e = hipMalloc(&P->d_desc, P->host_desc.size() * sizeof(bjob));
+  std::vector<bjob> host;
+  std::vector<char> tiled;
+  bjob* d_desc = nullptr;
+  std::uint32_t* d_jtab = nullptr;
+  std::uint64_t tiles = 0;
+
+  void release()
+  {
+    if (d_desc) {
+      (void)hipFree(d_desc);
+      d_desc = nullptr;
+    }
+    if (d_jtab) {
+      (void)hipFree(d_jtab);
+      d_jtab = nullptr;
+    }
+  }
+
+  int build(const frsz2_job* jobs, std::size_t n, const class_shapes* sh)
+  {
+    release();
+    host.clear();
+    tiles = build_descriptors(jobs, n, host, tiled, sh);
+    if (host.empty()) {
+      return FRSZ2_OK;
+    }
+    hipError_t e = hipMalloc(&d_desc, host.size() * sizeof(bjob));
     if (e != hipSuccess) {
       return hip_status(e, "hipMalloc");
     }
-    e = hipMemcpy(P->d_desc, P->host_desc.data(), P->host_desc.size() * sizeof(bjob), hipMemcpyHostToDevice);
+    e = hipMemcpy(d_desc, host.data(), host.size() * sizeof(bjob), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       return hip_status(e, "hipMemcpy");
     }
-    if (P->total_tiles > 0) {
+    if (tiles > 0) {
       // the tile's job by table (4 bytes per tile of at least 1 KiB of data)
       // instead of a search over thousands of jobs
-      const std::vector<std::uint32_t> tab = tile_table(P->host_desc, P->total_tiles, true);
-      e = hipMalloc(&P->d_jtab, tab.size() * sizeof(std::uint32_t));
+      const std::vector<std::uint32_t> tab = tile_table(host, tiles, true);
+      e = hipMalloc(&d_jtab, tab.size() * sizeof(std::uint32_t));
       if (e != hipSuccess) {
         return hip_status(e, "hipMalloc");
       }
-      e = hipMemcpy(P->d_jtab, tab.data(), tab.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice);
+      e = hipMemcpy(d_jtab, tab.data(), tab.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice);
       if (e != hipSuccess) {
         return hip_status(e, "hipMemcpy");
       }
     }
+    return FRSZ2_OK;
   }
+};
+
+// The tile shapes a configuration runs: its own under one workgroup per
+// tile, the built-in ones under the persistent schedule.
+class_shapes
+effective_shapes(const frsz2_batch_config& c)
+{
+  class_shapes sh{};
+  if (c.schedule == FRSZ2_SCHEDULE_TILES) {
+    std::copy(c.values_per_thread, c.values_per_thread + FRSZ2_BATCH_CLASSES, sh.vpt);
+    std::copy(c.workgroup_size, c.workgroup_size + FRSZ2_BATCH_CLASSES, sh.wgs);
+  }
+  return sh;
+}
+
+bool
+same_shapes(const class_shapes& a, const class_shapes& b)
+{
+  for (int c = 0; c < FRSZ2_BATCH_CLASSES; ++c) {
+    int va = 0, wa = 0, vb = 0, wb = 0;
+    class_shape(c, &a, va, wa);
+    class_shape(c, &b, vb, wb);
+    if (va != vb || wa != wb) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// build and upload the descriptors and the tile table for the tile shapes sh
+int
+upload_descriptors(frsz2_batch_plan* P, const class_shapes& sh)
+{
+  device_descriptors dd;
+  dd.d_desc = P->d_desc;
+  dd.d_jtab = P->d_jtab;
+  P->d_desc = nullptr;
+  P->d_jtab = nullptr;
+  P->uploaded = false;
+  const int rc = dd.build(P->jobs.data(), P->jobs.size(), &sh);
+  if (rc) {
+    dd.release();
+    return rc;
+  }
+  P->host_desc = std::move(dd.host);
+  P->tiled = std::move(dd.tiled);
+  P->d_desc = dd.d_desc;
+  P->d_jtab = dd.d_jtab;
+  P->total_tiles = dd.tiles;
+  P->ndesc = static_cast<int>(P->host_desc.size());
+  P->shapes = sh;
+  P->uploaded = true;
   return FRSZ2_OK;
 }
 
@@ -600,8 +814,13 @@ enqueue(frsz2_batch_plan* P, hipStream_t s)
     }
     return FRSZ2_OK;
   }
+  if (!P->uploaded) {
+    // a configuration whose descriptors could not be built left none
+    return set_error(FRSZ2_EHIP, "batch plan has no descriptors (its last configuration failed)");
+  }
   const batch_schedule sc{ P->cfg.schedule, P->cfg.workgroups, P->cfg.xcd_order };
-  const hipError_t e = launch_batch(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, sc, s, P->d_jtab);
+  const hipError_t e =
+    launch_batch(P->d_desc, P->host_desc.data(), P->ndesc, P->total_tiles, sc, s, P->d_jtab, &P->shapes);
   if (e != hipSuccess) {
     return hip_status(e, "batch launch");
   }
@@ -662,17 +881,36 @@ capture_graph(frsz2_batch_plan* P)
   return rc;
 }
 
+// Takes configuration c (checked first: a bad tile shape leaves the plan as
+// it was), rebuilds the descriptors when its tile shapes differ from those
+// they were built with, and re-captures the graph.
 int
 apply_config(frsz2_batch_plan* P, const frsz2_batch_config& c)
 {
-  P->cfg = c;
-  if (P->cfg.schedule != FRSZ2_SCHEDULE_PERSISTENT) {
-    P->cfg.schedule = FRSZ2_SCHEDULE_TILES;
+  frsz2_batch_config n = c;
+  if (n.schedule != FRSZ2_SCHEDULE_PERSISTENT) {
+    n.schedule = FRSZ2_SCHEDULE_TILES;
   }
-  if (P->cfg.workgroups < 0) {
-    P->cfg.workgroups = 0;
+  if (n.workgroups < 0) {
+    n.workgroups = 0;
   }
-  P->cfg.xcd_order = P->cfg.xcd_order ? 1 : 0;
+  n.xcd_order = n.xcd_order ? 1 : 0;
+  class_shapes given{};
+  std::copy(n.values_per_thread, n.values_per_thread + FRSZ2_BATCH_CLASSES, given.vpt);
+  std::copy(n.workgroup_size, n.workgroup_size + FRSZ2_BATCH_CLASSES, given.wgs);
+  int rc = validate_shapes(given);
+  if (rc) {
+    return rc;
+  }
+  P->cfg = n;
+  const class_shapes sh = effective_shapes(n);
+  if (!P->uploaded || !same_shapes(sh, P->shapes)) {
+    drop_graph(P);
+    rc = upload_descriptors(P, sh);
+    if (rc) {
+      return rc;
+    }
+  }
   return capture_graph(P);
 }
 
@@ -711,11 +949,7 @@ frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_config
   }
   auto* P = new frsz2_batch_plan();
   P->jobs.assign(jobs, jobs + njobs);
-  int rc = upload_descriptors(P);
-  if (rc) {
-    destroy(P);
-    return rc;
-  }
+  int rc = FRSZ2_OK;
   frsz2_batch_config c{};
   if (cfg) {
     c = *cfg;
@@ -789,9 +1023,26 @@ frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, fr
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     return set_error(FRSZ2_EHIP, "hipEventCreate failed");
   }
-  // the tile schedules: one workgroup per tile or a persistent grid, each in
-  // workgroup order or XCD order
-  frsz2_batch_config best = plan->cfg;
+  // time of reps runs of `run` after one warm-up, in ms
+  auto timed = [&](auto&& run, float& ms) -> int {
+    int rc = run();
+    if (rc) {
+      return rc;
+    }
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == FRSZ2_OK; ++r) {
+      rc = run();
+    }
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      return set_error(FRSZ2_EHIP, "hipEventSynchronize failed");
+    }
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return rc;
+  };
+  // 1. the tile schedules, built-in tile shapes: one workgroup per tile or a
+  // persistent grid, each in workgroup order or XCD order
+  frsz2_batch_config best{};
   float best_ms = -1.0f;
   int rc = FRSZ2_OK;
   for (int sched = 0; sched < 2 && rc == FRSZ2_OK; ++sched) {
@@ -806,29 +1057,79 @@ frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, fr
       if (rc) {
         break;
       }
-      rc = enqueue(plan, s); // warm-up
-      if (rc) {
-        break;
-      }
-      (void)hipEventRecord(e0, s);
-      for (int r = 0; r < reps && rc == FRSZ2_OK; ++r) {
-        rc = enqueue(plan, s);
-      }
-      (void)hipEventRecord(e1, s);
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        rc = set_error(FRSZ2_EHIP, "hipEventSynchronize failed");
-        break;
-      }
       float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (best_ms < 0.0f || ms < best_ms) {
+      rc = timed([&] { return enqueue(plan, s); }, ms);
+      if (rc == FRSZ2_OK && (best_ms < 0.0f || ms < best_ms)) {
         best_ms = ms;
         best = c;
       }
     }
   }
+  // 2. one workgroup per tile: each class's tile shape, on that class's jobs
+  // alone (the jobs the built-in shapes tile; a shape is a candidate when
+  // its tile holds all of their blocks)
+  if (rc == FRSZ2_OK && best.schedule == FRSZ2_SCHEDULE_TILES) {
+    rc = apply_config(plan, best);
+    const std::vector<char> tiled = plan->tiled; // under the built-in shapes
+    const batch_schedule sc{ 0, 0, best.xcd_order };
+    for (int cls = 0; cls < FRSZ2_BATCH_CLASSES && rc == FRSZ2_OK; ++cls) {
+      std::vector<frsz2_job> sub;
+      std::uint64_t maxB = 0;
+      for (std::size_t i = 0; i < plan->jobs.size(); ++i) {
+        if (tiled[i] && job_class(plan->jobs[i]) == cls) {
+          sub.push_back(plan->jobs[i]);
+          maxB = plan->jobs[i].block > maxB ? plan->jobs[i].block : maxB;
+        }
+      }
+      if (sub.empty()) {
+        continue;
+      }
+      const bool two = cls_dtype(cls) == 1 && cls_ui(cls) >= 1;
+      float cls_best = -1.0f;
+      for (int vpt : { 2, 4, 8, 16 }) {
+        for (int wgs : { 64, 256 }) {
+          if ((vpt == 2 && !two) || static_cast<std::uint64_t>(vpt) * static_cast<std::uint64_t>(wgs) < maxB) {
+            continue;
+          }
+          class_shapes sh{};
+          sh.vpt[cls] = static_cast<std::uint8_t>(vpt);
+          sh.wgs[cls] = static_cast<std::uint16_t>(wgs);
+          device_descriptors dd;
+          rc = dd.build(sub.data(), sub.size(), &sh);
+          float ms = 0.0f;
+          if (rc == FRSZ2_OK) {
+            rc = timed(
+              [&] {
+                const hipError_t e = launch_batch(dd.d_desc, dd.host.data(), static_cast<int>(dd.host.size()), dd.tiles,
+                                                  sc, s, dd.d_jtab, &sh);
+                return e == hipSuccess ? FRSZ2_OK : hip_status(e, "batch launch");
+              },
+              ms);
+          }
+          dd.release();
+          if (rc) {
+            break;
+          }
+          if (cls_best < 0.0f || ms < cls_best) {
+            cls_best = ms;
+            int bv = 0;
+            int bw = 0;
+            class_shape(cls, nullptr, bv, bw);
+            best.values_per_thread[cls] = static_cast<unsigned char>(vpt == bv ? 0 : vpt);
+            best.workgroup_size[cls] = static_cast<unsigned short>(wgs == bw ? 0 : wgs);
+          }
+        }
+        if (rc) {
+          break;
+        }
+      }
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (best_ms < 0.0f) {
+    best = plan->cfg; // no schedule could be timed: keep the configuration
+  }
   best.use_graph = use_graph;
   const int rc2 = apply_config(plan, best);
   if (rc == FRSZ2_OK) {

@@ -252,6 +252,14 @@ enum
                                    once striding over the class's tiles */
 };
 
+/* Kernel classes of the tiled compress and decompress jobs of a batch:
+ * class = op * 14 + slot, op = FRSZ2_OP_COMPRESS (0) or FRSZ2_OP_DECOMPRESS
+ * (1); slot = 2 * u + packed for float32 (u = 0, 1, 2 for the storage width
+ * U = 8, 16, 32) and 6 + 2 * u + packed for float64 (u = 0..3 for U = 8..64),
+ * where U is the smallest of 8, 16, 32, 64 bits that holds `bits` and
+ * packed = (bits < U).  Each class runs one grouped kernel. */
+#define FRSZ2_BATCH_CLASSES 28
+
 typedef struct frsz2_batch_config
 {
   int schedule;        /* FRSZ2_SCHEDULE_TILES or FRSZ2_SCHEDULE_PERSISTENT
@@ -265,6 +273,15 @@ typedef struct frsz2_batch_config
                           frsz2_decompress / frsz2_gather launch per job, in
                           job order (the per-array baseline the batch is
                           measured against) */
+  /* Tile shape of each kernel class under FRSZ2_SCHEDULE_TILES (the
+     persistent schedule runs the built-in shapes): values per thread (4, 8
+     or 16; 2 as well for float64 with U >= 16) and threads per workgroup
+     (64 or 256); 0 = the class's built-in value (the plain kernels'
+     measured variant).  A job whose block size B exceeds its class's tile
+     (threads x values per thread) runs as a plain launch.  The shape changes
+     only how the work is split, never the bytes written. */
+  unsigned char values_per_thread[FRSZ2_BATCH_CLASSES];
+  unsigned short workgroup_size[FRSZ2_BATCH_CLASSES];
 } frsz2_batch_config;
 
 typedef struct frsz2_batch_plan frsz2_batch_plan;
@@ -277,10 +294,14 @@ int frsz2_batch_create(const frsz2_job* jobs, size_t njobs, const frsz2_batch_co
 /* Run every job of the plan, stream-ordered.  Launches of one plan that
  * overlap in time write the same outputs concurrently, so order them. */
 int frsz2_batch_launch(frsz2_batch_plan* plan, frsz2_stream_t stream);
-/* Time the tile schedules on the plan's jobs (one workgroup per tile and
- * persistent, each in workgroup and XCD order; each launched reps times on
- * stream, outputs overwritten with identical bytes), keep the fastest and
- * re-capture; the choice is written to *chosen if not NULL. */
+/* Autotune the plan on its own jobs (each candidate launched reps times on
+ * stream, outputs overwritten with identical bytes): first the tile
+ * schedule (one workgroup per tile and persistent, each in workgroup and XCD
+ * order, built-in tile shapes); then, when one workgroup per tile wins, each
+ * class's tile shape (values per thread x workgroup size, every shape whose
+ * tile holds the class's blocks), timed on that class's jobs alone.  Keeps
+ * the fastest and re-captures; the choice is written to *chosen if not
+ * NULL. */
 int frsz2_batch_autotune(frsz2_batch_plan* plan, frsz2_stream_t stream, int reps, frsz2_batch_config* chosen);
 int frsz2_batch_get_config(const frsz2_batch_plan* plan, frsz2_batch_config* cfg);
 int frsz2_batch_set_config(frsz2_batch_plan* plan, const frsz2_batch_config* cfg);

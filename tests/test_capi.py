@@ -164,6 +164,27 @@ def test_batch_rejects_undersized_outputs():
     assert BatchConfig  # the structure the plan functions take
 
 
+def test_batch_rejects_bad_tile_shapes():
+    """Per-class tile shapes are checked before any device call: values per
+    thread 4, 8, 16 (2 only for float64 classes with U >= 16), workgroup size
+    64 or 256, 0 = built-in."""
+    from frsz_amd.batch import BATCH_CLASSES, BatchConfig, job_class
+
+    import torch
+
+    h = ctypes.c_void_p()
+    f32_u16 = job_class(0, torch.float32, 16)
+    f64_u32 = job_class(1, torch.float64, 21)
+    assert (f32_u16, f64_u32) == (2, 14 + 6 + 2 * 2 + 1)
+    for cls, vpt, wgs, msg in [(f32_u16, 3, 0, "values per thread 3"), (f32_u16, 2, 0, "values per thread 2"),
+                               (f64_u32, 32, 0, "values per thread 32"), (BATCH_CLASSES - 1, 0, 128, "workgroup size 128")]:
+        cfg = BatchConfig.from_dict({"tile_shapes": {cls: [vpt, wgs]}})
+        assert capi.lib.frsz2_batch_create(None, 0, ctypes.byref(cfg), ctypes.byref(h)) == capi.EARG
+        assert msg in capi.last_error() and f"class {cls}:" in capi.last_error()
+    cfg = BatchConfig.from_dict({"tile_shapes": {f64_u32: [2, 64], f32_u16: [16, 256]}})
+    assert cfg.as_dict()["tile_shapes"] == {f32_u16: [16, 256], f64_u32: [2, 64]}
+
+
 def test_batch_job_helpers_check_tensors():
     import torch
 

@@ -160,6 +160,52 @@ def test_graph_replays_with_eager_work_between(fz, port):
         del junk
 
 
+SHAPES = [(v, w) for v in (2, 4, 8, 16) for w in (64, 256)]
+
+
+@pytest.mark.parametrize("vpt,wgs", SHAPES, ids=[f"{v}x{w}" for v, w in SHAPES])
+def test_tile_shapes_match_oracle(fz, port, vpt, wgs):
+    """Every class present in one tile shape (values per thread x workgroup
+    size; float32 classes take 4 values per thread where 2 is asked): the
+    bytes stay the oracle's, and jobs whose block exceeds the tile run as
+    plain launches.  The plan then switches to the built-in shapes, which
+    rebuilds its descriptors, and runs again."""
+    fz_mod, batch = fz
+    specs = make_arrays(5, count=40)
+    ref_s, ref_v = oracle_outputs(port, specs)
+
+    def shapes(op):
+        d = {}
+        for x, b, B in specs:
+            two = x.dtype == torch.float64 and b > 8  # U >= 16
+            d[batch.job_class(op, x.dtype, b)] = [vpt if vpt != 2 or two else 4, wgs]
+        return d
+
+    outs = [torch.full_like(s, 0x5A) for s in ref_s]
+    plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)],
+                           {"tile_shapes": shapes(0)})
+    assert plan.config["tile_shapes"] == shapes(0)
+    vals = [torch.full_like(x, float("nan")) for x, _, _ in specs]
+    dplan = batch.BatchPlan([batch.decompress_job(s, v, b, B) for (x, b, B), s, v in zip(specs, ref_s, vals)],
+                            {"tile_shapes": shapes(1), "use_graph": 0})
+    for rnd in range(2):
+        print(f"tile shape {vpt}x{wgs} round {rnd}", flush=True)
+        for o in outs:
+            o.fill_(0x5A)
+        for v in vals:
+            v.fill_(float("nan"))
+        plan.launch()
+        dplan.launch()
+        torch.cuda.synchronize()
+        for i, ((x, b, B), o, r) in enumerate(zip(specs, outs, ref_s)):
+            assert torch.equal(o, r), f"round {rnd} job {i}: dtype={x.dtype} bits={b} B={B} n={x.numel()}"
+        for i, (v, r) in enumerate(zip(vals, ref_v)):
+            assert torch.equal(v.view(torch.uint8), r.view(torch.uint8)), f"round {rnd} decompress job {i}"
+        plan.config = {"tile_shapes": {}}
+        dplan.config = {"tile_shapes": {}}
+        assert plan.config["tile_shapes"] == {}
+
+
 def test_autotune_keeps_bytes(fz, port):
     fz_mod, batch = fz
     specs = make_arrays(3, count=40)
@@ -167,8 +213,12 @@ def test_autotune_keeps_bytes(fz, port):
     outs = [torch.empty_like(s) for s in ref_s]
     plan = batch.BatchPlan([batch.compress_job(x, o, b, B) for (x, b, B), o in zip(specs, outs)])
     chosen = plan.autotune(reps=1)
-    # whichever candidate wins the timing, the plan keeps a valid schedule and the bytes
+    # whichever candidate wins the timing, the plan keeps a valid schedule,
+    # valid tile shapes and the bytes
     assert chosen["schedule"] in (0, 1) and chosen["xcd_order"] in (0, 1)
+    for cls, (vpt, wgs) in chosen["tile_shapes"].items():
+        assert chosen["schedule"] == 0 and 0 <= cls < 14
+        assert vpt in (0, 2, 4, 8, 16) and wgs in (0, 64, 256)
     assert chosen == plan.config
     for o in outs:
         o.zero_()
