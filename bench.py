@@ -585,6 +585,31 @@ def bench_ceiling(mib=1024):
             "vectors per lane, non-temporal hint and tile order"}
 
 
+def bench_random_ceiling(indices=10_000_000, values=100_000_037):
+    """The random-access ceiling of this device from tests/native/stream_ceiling
+    --random (a child process): the gather's memory pattern (an 8-byte index,
+    a random 2-byte code of a 2n-byte payload and its block's 2-byte exponent,
+    a 4-byte store per index) without the decode.  Returns the per-variant
+    times, or None when the program is not built."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "native", "stream_ceiling")
+    if not os.path.exists(exe):
+        return None
+    res = subprocess.run([exe, "--random", str(indices), "--values", str(values)], stdout=subprocess.PIPE,
+                         stderr=subprocess.STDOUT, text=True, timeout=300)
+    if res.returncode != 0:
+        return {"error": res.stdout[-500:]}
+    rows = [json.loads(ln) for ln in res.stdout.splitlines() if ln.startswith("{")]
+    us = {}
+    for r in rows:
+        key = r["loads"] + (" (non-temporal code)" if r["nt_code"] else "")
+        us[key] = r["us"]
+    best = min(r["us"] for r in rows if r["loads"] == "code+exponent")
+    return {"indices": indices, "values": values, "us": us, "best_code_exponent_us": best,
+            "source": "tests/native/stream_ceiling.hip --random"}
+
+
 def _best_of(fn, reps=5):
     """Wall seconds of fn: one untimed warm-up call, then the best of `reps`."""
     fn()
@@ -863,6 +888,14 @@ def main():
     if not args.no_edge:
         print("bench: edge-case sweep", file=sys.stderr, flush=True)
         line["edge"], edge_host = bench_edge()
+        if not args.no_copy:
+            print("bench: random-access ceiling", file=sys.stderr, flush=True)
+            rc = bench_random_ceiling(line["edge"]["gather"]["indices"], line["edge"]["n"])
+            if rc is not None:
+                line["edge"]["gather"]["random_access_ceiling"] = rc
+                if "best_code_exponent_us" in rc:
+                    line["edge"]["gather"]["frac_of_random_ceiling"] = round(
+                        rc["best_code_exponent_us"] / line["edge"]["gather"]["us"], 4)
     if not args.no_generic:
         print("bench: block sizes that are not powers of two, unaligned streams", file=sys.stderr, flush=True)
         line["generic"] = bench_generic()

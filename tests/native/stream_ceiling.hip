@@ -21,6 +21,17 @@
 // Prints one JSON line per (pattern, workgroup size, UNR, NT, order) with
 // the best of three rounds of ten launches, event-timed, and exits non-zero
 // if a copy's or a mix's bytes are wrong.
+//
+// Random-access ceiling: stream_ceiling --random M [--values N]
+// The memory pattern of a random-access decode of M uniform indices into a
+// float32 stream of N values (default 100000037) at bits 16, B 32, without
+// the decode: per index an 8-byte index load (consecutive), a 2-byte load
+// of a random code in the 2 N-byte payload and of its block's 2-byte
+// exponent in the N/16-byte exponent array, and a 4-byte store of their
+// XOR (consecutive).  Variants: both loads, code load only, exponent load
+// only; plain or non-temporal code loads.  One JSON line per variant (best
+// of three rounds of ten launches); the stored words are checked against
+// the host's.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -148,6 +159,155 @@ struct result
   double gbs;
 };
 
+// ---- random-access ceiling
+
+// index i of the random pattern: a 64-bit mix of i, reduced to [0, n)
+__host__ __device__ inline std::uint64_t
+random_index(std::uint64_t i, std::uint64_t n)
+{
+  std::uint64_t h = (i + 1) * 0x9E3779B97F4A7C15ull;
+  h ^= h >> 31;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 29;
+  return h % n;
+}
+
+__global__ void
+k_fill_indices(std::uint64_t* idx, std::uint64_t m, std::uint64_t n)
+{
+  for (std::uint64_t i = blockIdx.x * std::uint64_t(blockDim.x) + threadIdx.x; i < m;
+       i += std::uint64_t(gridDim.x) * blockDim.x)
+    idx[i] = random_index(i, n);
+}
+
+// 16-bit word k of the filled payload / exponent arrays (k_fill's pattern)
+__host__ __device__ inline std::uint16_t
+pattern_half(std::uint64_t k)
+{
+  const auto lo = static_cast<std::uint32_t>(k / 8);
+  const std::uint32_t w[4] = { lo * 0x9E3779B1u, lo, ~lo, lo ^ 0x55AA55AAu };
+  const std::uint32_t d = w[(k / 2) % 4];
+  return static_cast<std::uint16_t>(k % 2 ? d >> 16 : d);
+}
+
+// CODE / EXP: which of the two random loads run; NTC: non-temporal code loads
+template<bool CODE, bool EXP, bool NTC>
+__global__ __launch_bounds__(256) void
+k_random(const std::uint64_t* __restrict__ idx, std::uint64_t m, const std::uint16_t* __restrict__ code,
+         const std::uint16_t* __restrict__ exps, std::uint32_t* __restrict__ out)
+{
+  const std::uint64_t i = blockIdx.x * std::uint64_t(256) + threadIdx.x;
+  if (i >= m)
+    return;
+  const std::uint64_t j = idx[i];
+  std::uint32_t v = 0;
+  if constexpr (CODE)
+    v = NTC ? __builtin_nontemporal_load(code + j) : code[j];
+  if constexpr (EXP)
+    v ^= static_cast<std::uint32_t>(exps[j >> 5]) << 16;
+  out[i] = v;
+}
+
+template<bool CODE, bool EXP, bool NTC>
+int
+run_random(const char* name, const std::uint64_t* idx, std::uint64_t m, std::uint64_t n, const std::uint16_t* code,
+           const std::uint16_t* exps, std::uint32_t* out, const std::vector<std::uint64_t>& hidx, hipEvent_t e0,
+           hipEvent_t e1)
+{
+  const unsigned grid = static_cast<unsigned>((m + 255) / 256);
+  auto launch = [&] {
+    hipLaunchKernelGGL((k_random<CODE, EXP, NTC>), dim3(grid), dim3(256), 0, nullptr, idx, m, code, exps, out);
+  };
+  for (int i = 0; i < 2; ++i)
+    launch();
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  float best = 0.f;
+  for (int round = 0; round < 3; ++round) {
+    HIP_CHECK(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < 10; ++i)
+      launch();
+    HIP_CHECK(hipEventRecord(e1, nullptr));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (round == 0 || ms < best)
+      best = ms;
+  }
+  // check the first and last 4096 stored words
+  int fails = 0;
+  const std::uint64_t k = m < 4096 ? m : 4096;
+  std::vector<std::uint32_t> got(k);
+  for (int part = 0; part < 2; ++part) {
+    const std::uint64_t first = part == 0 ? 0 : m - k;
+    HIP_CHECK(hipMemcpy(got.data(), out + first, k * 4, hipMemcpyDeviceToHost));
+    for (std::uint64_t q = 0; q < k; ++q) {
+      const std::uint64_t j = hidx[part * k + q];
+      std::uint32_t want = 0;
+      if (CODE)
+        want = pattern_half(j);
+      if (EXP)
+        want ^= static_cast<std::uint32_t>(pattern_half(j >> 5)) << 16;
+      if (got[q] != want) {
+        ++fails;
+        break;
+      }
+    }
+  }
+  const double us = best / 10.0 * 1e3;
+  std::printf("{\"pattern\": \"random\", \"loads\": \"%s\", \"nt_code\": %d, \"indices\": %llu, \"values\": %llu, "
+              "\"us\": %.1f, \"gindex_s\": %.2f, \"ok\": %d}\n",
+              name, int(NTC), static_cast<unsigned long long>(m), static_cast<unsigned long long>(n), us,
+              double(m) / us * 1e-3, fails == 0);
+  std::fflush(stdout);
+  return fails;
+}
+
+int
+random_ceiling(std::uint64_t m, std::uint64_t n)
+{
+  const std::uint64_t nexp = (n + 31) / 32;
+  // payload: n codes; exponents: nexp words; both padded to whole vectors
+  const std::uint64_t pvec = (2 * n + 15) / 16;
+  const std::uint64_t evec = (2 * nexp + 15) / 16;
+  uint4 *code = nullptr, *exps = nullptr;
+  std::uint64_t* idx = nullptr;
+  std::uint32_t* out = nullptr;
+  HIP_CHECK(hipMalloc(&code, pvec * 16));
+  HIP_CHECK(hipMalloc(&exps, evec * 16));
+  HIP_CHECK(hipMalloc(&idx, m * 8));
+  HIP_CHECK(hipMalloc(&out, m * 4));
+  hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, nullptr, code, pvec);
+  hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, nullptr, exps, evec);
+  hipLaunchKernelGGL(k_fill_indices, dim3(4096), dim3(256), 0, nullptr, idx, m, n);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  const std::uint64_t k = m < 4096 ? m : 4096;
+  std::vector<std::uint64_t> hidx(2 * k);
+  for (std::uint64_t q = 0; q < k; ++q) {
+    hidx[q] = random_index(q, n);
+    hidx[k + q] = random_index(m - k + q, n);
+  }
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  const auto* c16 = reinterpret_cast<const std::uint16_t*>(code);
+  const auto* x16 = reinterpret_cast<const std::uint16_t*>(exps);
+  int fails = 0;
+  fails += run_random<true, true, false>("code+exponent", idx, m, n, c16, x16, out, hidx, e0, e1);
+  fails += run_random<true, true, true>("code+exponent", idx, m, n, c16, x16, out, hidx, e0, e1);
+  fails += run_random<true, false, false>("code", idx, m, n, c16, x16, out, hidx, e0, e1);
+  fails += run_random<true, false, true>("code", idx, m, n, c16, x16, out, hidx, e0, e1);
+  fails += run_random<false, true, false>("exponent", idx, m, n, c16, x16, out, hidx, e0, e1);
+  HIP_CHECK(hipFree(code));
+  HIP_CHECK(hipFree(exps));
+  HIP_CHECK(hipFree(idx));
+  HIP_CHECK(hipFree(out));
+  if (fails)
+    std::printf("FAIL %d random-access checks\n", fails);
+  return fails ? 1 : 0;
+}
+
 template<int R, int W, int UNR, bool NT>
 result
 run(const uint4* src, uint4* dst, uint4* sink, std::uint64_t side_bytes, int wg, int xcd, hipEvent_t e0,
@@ -238,9 +398,23 @@ int
 main(int argc, char** argv)
 {
   std::uint64_t mib = 1024;
+  std::uint64_t random_m = 0;
+  std::uint64_t random_n = 100000037;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--mib") && i + 1 < argc)
       mib = std::strtoull(argv[++i], nullptr, 10);
+    else if (!std::strcmp(argv[i], "--random") && i + 1 < argc)
+      random_m = std::strtoull(argv[++i], nullptr, 10);
+    else if (!std::strcmp(argv[i], "--values") && i + 1 < argc)
+      random_n = std::strtoull(argv[++i], nullptr, 10);
+  }
+  if (random_m > 0) {
+    // one launch of one thread per index: at most 2^32 work-items
+    if (random_m > (std::uint64_t{ 1 } << 31) || random_n < 32 || random_n > (std::uint64_t{ 1 } << 34)) {
+      std::fprintf(stderr, "--random must be in 1..2^31 and --values in 32..2^34\n");
+      return 2;
+    }
+    return random_ceiling(random_m, random_n);
   }
   // the largest tile is 2 units of 256 * 4 vectors (32 KiB): sizes in MiB divide evenly
   if (mib < 1 || mib > 16384) {
