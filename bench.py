@@ -543,11 +543,31 @@ def bench_serve(seconds, rate, budget_gib, seed=5, verify_every=499, server_opti
     per = {names[k]: {"count": int((kind == k).sum()), "p50_ms": pct(lat[kind == k], 50),
                       "p99_ms": pct(lat[kind == k], 99)} for k in (0, 1, 2)}
     payload = float(np.sum(np.where(kind == 2, m * vs, n * vs)))
+    # each request's own HBM bytes (values + stream; a gather: index, value,
+    # exponent and code per element) and the time they take at peak
+    # bandwidth: the floor below which no scheduling could bring its latency
+    nb = (n + B - 1) // B
+    U = bits  # bits 16 / 32 are stored unpacked
+    X = nb * U // 8
+    stream_bytes = X + 256 - X % 256 + nb * (B * bits // 8)
+    own = np.where(kind == 2, m * (8 + vs + 2 * (U // 8)), n * vs + stream_bytes).astype(np.float64)
+    floor_ms = own / 8e12 * 1e3
+    small, large = own <= (1 << 20), own >= (64 << 20)
+    per_size = {"floor_at_peak_p99_ms": pct(floor_ms, 99),
+                "p99_minus_floor_ms": pct(lat - floor_ms, 99),
+                "p99_requests_to_1mib_ms": pct(lat[small], 99), "requests_to_1mib": int(small.sum()),
+                "p99_requests_from_64mib_ms": pct(lat[large], 99), "requests_from_64mib": int(large.sum()),
+                # where the requests to 1 MiB spend their p99: queued -> admitted,
+                # admitted -> launched, launched -> completion seen
+                "stages_p99_requests_to_1mib_ms": {
+                    "admit": pct(((res["t_admit"] - res["t_arrival"]) * 1e3)[small], 99),
+                    "launch": pct(((res["t_launch"] - res["t_admit"]) * 1e3)[small], 99),
+                    "run": pct(((res["t_done"] - res["t_launch"]) * 1e3)[small], 99)}}
     del src, idx_all
     torch.cuda.empty_cache()
     return {"requests": nreq, "rate_offered": rate, "seconds": seconds, "hbm_budget_gib": budget_gib,
             "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "p999_ms": pct(lat, 99.9), "max_ms": pct(lat, 100),
-            "per_op": per,
+            "per_op": per, "by_size": per_size,
             # median of each stage: queued -> admitted (output carved from the
             # pool), admitted -> launched, launched -> completion seen
             "stages_p50_ms": {"admit": pct((res["t_admit"] - res["t_arrival"]) * 1e3, 50),
@@ -909,7 +929,7 @@ def main():
             load.append({k: r[k] for k in ("rate_offered", "seconds", "requests", "p50_ms", "p99_ms", "p999_ms",
                                            "throughput_req_s", "payload_gbs", "all_ok", "bit_exact_sample",
                                            "mean_requests_per_launch", "pool_peak_gib", "max_queue",
-                                           "stages_p50_ms")})
+                                           "stages_p50_ms", "by_size")})
         if load:
             line["serve_load"] = load
     if not args.no_verify or not args.no_cpu:

@@ -19,6 +19,11 @@
 //   host memory).  Up to
 //   `slots` launches run on the device at once; requests that arrive while
 //   they run are admitted into the next launch as slots and memory free.
+// * Large requests (at least `large_bytes` of HBM traffic) are batched only
+//   with other large requests: a launch completes as a whole, so a small
+//   request batched with a large one would wait for all of the large one's
+//   bytes.  Launches of large requests hold at most slots - 1 slots at once,
+//   so a small request always finds a slot that no large request holds.
 // * Completion: the scheduler polls each slot's event; requests complete,
 //   their outputs stay resident until released (or right away when the
 //   request asked to discard them), and a stream stays resident until the
@@ -156,6 +161,7 @@ struct slot
   std::vector<bjob> desc;
   std::vector<char> tiled;
   bool busy = false;
+  bool large = false; // the launch holds one large request
 };
 
 std::uint64_t
@@ -292,23 +298,44 @@ complete_slot(frsz2_server* S, slot& sl, int status)
   sl.busy = false;
 }
 
+// HBM bytes a request moves: its values and its stream (a gather: index,
+// value and the words it reads per element)
+std::uint64_t
+traffic_bytes(const frsz2_request& r, std::uint64_t out_bytes)
+{
+  const std::uint64_t vs = r.dtype == FRSZ2_FLOAT32 ? 4 : 8;
+  if (r.op == FRSZ2_OP_GATHER) {
+    return r.m * (8 + 3 * vs);
+  }
+  const std::uint64_t stream = r.op == FRSZ2_OP_COMPRESS ? out_bytes : layout::make(r.bits, r.block, r.n).total_bytes();
+  return r.n * vs + stream;
+}
+
 // admits queued requests into sl (called with the lock held); returns true
-// if anything was admitted
+// if anything was admitted.  A launch holds only small or only large
+// requests, whichever its first admitted request is; allow_large: the slot
+// may take large requests.
 bool
-admit(frsz2_server* S, slot& sl)
+admit(frsz2_server* S, slot& sl, bool allow_large)
 {
   sl.jobs.clear();
   sl.tickets.clear();
+  sl.large = false;
   const double now = seconds_since(S, clk::now());
   for (auto it = S->queue.begin(); it != S->queue.end();) {
     if (static_cast<int>(sl.jobs.size()) >= S->cfg.max_batch) {
       break;
     }
     record& q = S->recs[*it];
+    const bool large = traffic_bytes(q.r, q.out_bytes) >= S->cfg.large_bytes;
+    if (large ? !allow_large || (!sl.jobs.empty() && !sl.large) : sl.large) {
+      ++it; // the other kind of launch takes it
+      continue;
+    }
     const record* src = nullptr;
     if (q.r.op != FRSZ2_OP_COMPRESS) {
       auto s = S->recs.find(q.source);
-      if (s == S->recs.end() || s->second.state == FAILED || !s->second.out_live) {
+      if (s == S->recs.end() || s->second.state == FAILED || (s->second.state != QUEUED && !s->second.out_live)) {
         // the stream it reads is gone
         const std::uint64_t t = *it;
         it = S->queue.erase(it);
@@ -317,7 +344,8 @@ admit(frsz2_server* S, slot& sl)
         continue;
       }
       if (s->second.state != DONE) {
-        ++it; // stream not written yet
+        ++it; // stream not written yet (its compress still queued, as a large
+              // request waiting for a slot, or running)
         continue;
       }
       src = &s->second;
@@ -350,6 +378,7 @@ admit(frsz2_server* S, slot& sl)
     }
     sl.jobs.push_back(j);
     sl.tickets.push_back(*it);
+    sl.large = large;
     it = S->queue.erase(it);
   }
   return !sl.jobs.empty();
@@ -415,13 +444,19 @@ scheduler(frsz2_server* S)
       S->cv.notify_all();
     }
     if (!S->queue.empty()) {
+      int large_busy = 0;
+      for (const slot& sl : S->slots) {
+        large_busy += sl.busy && sl.large ? 1 : 0;
+      }
+      const int large_cap = S->cfg.slots > 1 ? S->cfg.slots - 1 : 1;
       for (slot& sl : S->slots) {
         if (sl.busy || S->queue.empty()) {
           continue;
         }
-        if (!admit(S, sl)) {
+        if (!admit(S, sl, large_busy < large_cap)) {
           break;
         }
+        large_busy += sl.large ? 1 : 0;
         sl.busy = true;
         any_busy = true;
         ++S->launches;
@@ -606,6 +641,9 @@ frsz2_server_create(const frsz2_server_config* cfg, frsz2_server** out)
   S->cfg.xcd_order = S->cfg.xcd_order ? 1 : 0;
   if (S->cfg.stall_ms == 0) {
     S->cfg.stall_ms = 10000;
+  }
+  if (S->cfg.large_bytes == 0) {
+    S->cfg.large_bytes = std::uint64_t{ 32 } << 20;
   }
   hipError_t e = hipMalloc(&S->pool, S->cfg.hbm_budget);
   if (e != hipSuccess) {

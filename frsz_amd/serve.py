@@ -30,7 +30,7 @@ REQ_DISCARD = 1
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("slots", ctypes.c_int), ("max_batch", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("workgroups", ctypes.c_int), ("xcd_order", ctypes.c_int),
-                ("stall_ms", ctypes.c_int)]
+                ("stall_ms", ctypes.c_int), ("large_bytes", ctypes.c_uint64)]
 
 
 class Request(ctypes.Structure):
@@ -90,8 +90,11 @@ def result_dict(r: Result) -> dict:
 
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, slots: int = 4, max_batch: int = 1024, schedule: int = 0,
-                 workgroups: int = 0, xcd_order: int = 0):
-        cfg = ServerConfig(hbm_budget, slots, max_batch, schedule, workgroups, xcd_order, 0)
+                 workgroups: int = 0, xcd_order: int = 0, large_bytes: int = 0):
+        """large_bytes: requests moving at least this many HBM bytes are
+        batched only with each other, on at most slots - 1 slots at once
+        (0 = 32 MiB, 2**64 - 1 = none)."""
+        cfg = ServerConfig(hbm_budget, slots, max_batch, schedule, workgroups, xcd_order, 0, large_bytes)
         h = ctypes.c_void_p()
         check(_L.frsz2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -348,6 +348,12 @@ typedef struct frsz2_server_config
   int stall_ms;        /* when nothing runs and the oldest queued request has
                           not fit the pool for this long, it fails with
                           FRSZ2_ECAPACITY (0 = 10000; negative = never) */
+  uint64_t large_bytes; /* a request that moves at least this many HBM bytes
+                           (values + stream; a gather: its elements) is large:
+                           it is batched only with other large requests, and
+                           launches of large requests hold at most slots - 1
+                           slots at once, so small requests always find one
+                           (0 = 32 MiB; UINT64_MAX = no request is large) */
 } frsz2_server_config;
 
 enum

@@ -135,3 +135,72 @@ def test_replay_trace_equals_oracle(mods, port):
         for tk in tickets:
             srv.release(int(tk))
         assert srv.stats()["pool_used"] == 0
+
+
+def test_large_requests_batch_apart(mods, port):
+    """Requests moving at least large_bytes are never batched with smaller
+    ones (a launch completes as a whole, so a small request batched with a
+    large one would wait for its bytes), and their launches hold at most
+    slots - 1 slots at once.  Every result equals the oracle's."""
+    fz, serve = mods
+    from frsz_amd.workloads import smooth_signal_torch
+
+    large_bytes = 1 << 20
+    rng = np.random.default_rng(9)
+    specs = []
+    for i in range(30):
+        big = i % 5 == 0
+        n = int(rng.integers(300_000, 500_000)) if big else int(rng.integers(100, 4000))
+        x = smooth_signal_torch(n, seed=700 + i, dtype=torch.float32)
+        specs.append((x, 16, 32, big))
+    rows = [(0, 0, bits, 0, B, x.numel(), x.data_ptr(), 0, 0, 0, 0.0) for x, bits, B, _ in specs]
+    trace = np.array(rows, dtype=serve.REQUEST_DTYPE)  # all released at once: admission batches what it can
+    with serve.Server(hbm_budget=1 << 30, slots=3, large_bytes=large_bytes) as srv:
+        res, tickets = srv.replay(trace)
+        assert np.all(res["status"] == 0)
+        launch = res["t_launch"]
+        for i, (x, bits, B, big) in enumerate(specs):
+            assert big == (x.numel() * 4 + fz.compressed_size(torch.float32, bits, B, x.numel()) >= large_bytes)
+            together = [j for j in range(len(specs)) if launch[j] == launch[i]]
+            assert all(specs[j][3] == big for j in together), f"request {i} shared a launch across sizes"
+            want = oracle_codec(port, x, bits, B)[0]
+            r = {k: res[i][k] for k in res.dtype.names}
+            assert torch.equal(srv.output(r, torch.uint8, want.numel()), want), i
+            srv.release(int(tickets[i]))
+        # at most slots - 1 = 2 large launches overlap at any time
+        big_launches = sorted({(float(res["t_launch"][i]), float(res["t_done"][i])) for i, s in enumerate(specs)
+                               if s[3]})
+        for t0, _ in big_launches:
+            assert sum(1 for a, b in big_launches if a <= t0 < b) <= 2
+
+
+def test_reader_waits_for_queued_large_source(mods, port):
+    """A decompress or gather whose stream's compress is still queued (a
+    large request waiting for one of the slots - 1 large slots) waits for it
+    instead of failing."""
+    fz, serve = mods
+    from frsz_amd.workloads import smooth_signal_torch
+
+    xs = [smooth_signal_torch(4_000_000 + 1000 * i, seed=800 + i, dtype=torch.float32) for i in range(3)]
+    idx = torch.arange(0, 4_000_000, 997, device="cuda", dtype=torch.int64)
+    rows = []
+    for i, x in enumerate(xs):
+        rows.append((0, 0, 16, 0, 32, x.numel(), x.data_ptr(), 0, 0, 0, 0.0))
+    for i, x in enumerate(xs):
+        rows.append((2, 0, 16, 0, 32, x.numel(), 0, i, idx.data_ptr(), idx.numel(), 0.0))
+        rows.append((1, 0, 16, 0, 32, x.numel(), 0, i, 0, 0, 0.0))
+    trace = np.array(rows, dtype=serve.REQUEST_DTYPE)
+    with serve.Server(hbm_budget=1 << 30, slots=2, large_bytes=1 << 20) as srv:
+        res, tickets = srv.replay(trace)
+        assert np.all(res["status"] == 0), res["status"]
+        for i, x in enumerate(xs):
+            s, y = oracle_codec(port, x, 16, 32)
+            r = {k: res[i][k] for k in res.dtype.names}
+            assert torch.equal(srv.output(r, torch.uint8, s.numel()), s)
+            rg = {k: res[3 + 2 * i][k] for k in res.dtype.names}
+            assert torch.equal(srv.output(rg, torch.float32, idx.numel()).view(torch.uint8), y[idx].view(torch.uint8))
+            rd = {k: res[4 + 2 * i][k] for k in res.dtype.names}
+            assert torch.equal(srv.output(rd, torch.float32, x.numel()).view(torch.uint8), y.view(torch.uint8))
+        for t in tickets:
+            if t:
+                srv.release(int(t))
