@@ -29,6 +29,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# Hardware queues for this process, read when HIP initialises.  The serving
+# runtime launches on 4 streams of its own, the host-memory pipeline keeps 3,
+# and torch has its default stream; with HIP's default of 4 queues, streams
+# share queues, and a small request's kernel can then wait in a queue behind
+# a large one.  8 gives every stream here a queue of its own (serving, 10 s
+# at 20k req/s: p50 0.027 -> 0.023 ms, p99 of requests up to 1 MiB 0.076 -> 0.068 ms).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 HBM_PEAK_GBS = 8000.0
 
 
