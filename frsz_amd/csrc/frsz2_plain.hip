@@ -508,20 +508,29 @@ effective_vpt(int vpt, std::uint64_t B, int wgs)
 
 // Tiled launch with the variant of the tuning table (values per thread,
 // non-temporal hint, tile order, staged decompress stores, workgroup size).
-// A block too large for a 64-thread tile takes the 256-thread one; B beyond
-// the largest tile, or unaligned pointers, take the generic kernels.
-// Workgroups of 64 threads are built with the non-temporal hint only.
+// A block too large for the tuned tile grows the tile: more values per
+// thread, and a 64-thread tile becomes the 256-thread one when no 64-thread
+// tile holds the block.  prefer_wide (decompress): a 64-thread tile becomes
+// the 256-thread one as soon as that needs fewer values per thread (float64
+// bits 32 decompress at B = 1024: 64 x 16 ran at 5.2 TB/s, 256 x 8 at 6.3;
+// compress measured better the other way, float32 bits 32 at B = 512: 6.5
+// against 6.0).  B beyond the largest tile, or unaligned pointers, take the
+// generic kernels.  Workgroups of 64 threads are built with the
+// non-temporal hint only.
 template<typename T, int U>
 int
-tiled_vpt(bool tiled, int vpt, std::uint64_t B, bool& small)
+tiled_vpt(bool tiled, int vpt, std::uint64_t B, bool& small, bool prefer_wide)
 {
   if (!tiled) {
     return 0;
   }
   int v = effective_vpt<T, U>(vpt, B, small ? 64 : WG);
-  if (v == 0 && small) {
-    small = false;
-    v = effective_vpt<T, U>(vpt, B, WG);
+  if (small) {
+    const int wide = effective_vpt<T, U>(vpt, B, WG);
+    if (v == 0 || (prefer_wide && wide != 0 && wide < v)) {
+      small = false;
+      v = wide;
+    }
   }
   return v;
 }
@@ -541,7 +550,7 @@ run_compress(const void* in, void* out, const layout& L, std::uint32_t zpad, boo
   }
   const variant v = get_variant(OP_COMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   bool small = v.small != 0;
-  int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small);
+  int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small, false);
   if (vpt > 0) {
 #define FRSZ2_CL(V, N, W) return launch_compress_tiled<T, U, V, N, W>(in, out, L, zpad, lb, v.linear, s)
     if (small) {
@@ -584,9 +593,13 @@ run_decompress(const void* in, void* out, const layout& L, bool tiled, bool stri
 {
   const variant v = get_variant(OP_DECOMPRESS, sizeof(T) == 4 ? FRSZ2_FLOAT32 : FRSZ2_FLOAT64, L.bits);
   bool small = v.small != 0;
-  int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small);
-  // strided: the decode that stores single values (any value alignment)
-  const int stage = strided ? 0 : v.stage;
+  int vpt = tiled_vpt<T, U>(tiled, v.vpt, L.B, small, true);
+  // strided: the decode that stores single values (any value alignment).
+  // A float64 tile grown to 16 values per thread to hold its block takes it
+  // too: the staged decode at 16 float64 values per thread (128 bytes of
+  // output per lane) measured 4.1-5.2 TB/s at B 512-4096, the strided one
+  // 6.0-6.1.
+  const int stage = strided || (sizeof(T) == 8 && vpt == 16 && v.vpt < 16) ? 0 : v.stage;
   if (vpt > 0) {
 #define FRSZ2_DL(V, N, S, W) return launch_decompress_tiled<T, U, V, N, S, W>(in, out, L, lb, v.linear, s)
     if (small) {

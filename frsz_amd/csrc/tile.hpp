@@ -48,6 +48,7 @@ struct tplan
 };
 
 typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef std::uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // LDS dwords a packed tile needs (payload image plus 3 dwords of slack for
 // the funnel reads / ORs that touch the dword after a code's last one)
@@ -706,6 +707,20 @@ decode_chunk(const typename fmt<T>::uw (&code)[VC],
     for (int v = 0; v < VC; ++v) {
       y[v] = decode_value_short<T>(code[v], e, bits);
     }
+    return;
+  }
+  // several blocks in the chunk (B < VC): the short form per value when
+  // every one of their exponents allows it
+  int emin = E[0];
+#pragma unroll
+  for (int v = 1; v < VC; ++v) {
+    emin = E[v] < emin ? E[v] : emin;
+  }
+  if (narrow && emin + fmt<T>::BIAS >= 0) {
+#pragma unroll
+    for (int v = 0; v < VC; ++v) {
+      y[v] = decode_value_short<T>(code[v], E[v], bits);
+    }
   } else {
 #pragma unroll
     for (int v = 0; v < VC; ++v) {
@@ -875,6 +890,50 @@ decompress_tile_strided(const std::uint8_t* __restrict__ in,
   }
 }
 
+// Block exponents of one chunk of VC values that holds NBK = VC >> LB whole
+// exponent blocks (B = 2^LB < VC): their NBK slots are contiguous in the
+// stream and aligned to their size (the chunk starts at a multiple of VC
+// values), so they come in with one load of NBK*U/8 <= 16 bytes instead of
+// one load per value.  E[v] is the exponent of value v's block.
+template<int U, int VC, int LB>
+__device__ __forceinline__ void
+chunk_exponents_vec(const std::uint8_t* __restrict__ in, std::uint64_t k0, int (&E)[VC])
+{
+  constexpr int NBK = VC >> LB;
+  constexpr int BYTES = NBK * (U / 8);
+  static_assert(BYTES >= 1 && BYTES <= 16, "one load of at most 16 bytes");
+  const std::uint8_t* p = in + k0 * static_cast<std::uint64_t>(U / 8);
+  std::uint32_t w[4] = { 0, 0, 0, 0 };
+  if constexpr (BYTES == 16) {
+    const u32x4 d = *reinterpret_cast<const u32x4*>(p);
+    w[0] = d.x;
+    w[1] = d.y;
+    w[2] = d.z;
+    w[3] = d.w;
+  } else if constexpr (BYTES == 8) {
+    const u32x2 d = *reinterpret_cast<const u32x2*>(p);
+    w[0] = d.x;
+    w[1] = d.y;
+  } else if constexpr (BYTES == 4) {
+    w[0] = *reinterpret_cast<const std::uint32_t*>(p);
+  } else if constexpr (BYTES == 2) {
+    w[0] = *reinterpret_cast<const std::uint16_t*>(p);
+  } else {
+    w[0] = *p;
+  }
+#pragma unroll
+  for (int v = 0; v < VC; ++v) {
+    constexpr int UB = U < 32 ? U : 32; // a 64-bit slot's exponent is in its low dword
+    const int bit = (v >> LB) * U;
+    const std::uint32_t d = w[bit >> 5] >> (bit & 31);
+    if constexpr (UB == 32) {
+      E[v] = static_cast<int>(d);
+    } else {
+      E[v] = static_cast<int>(d << (32 - UB)) >> (32 - UB); // sign-extend the U-bit slot
+    }
+  }
+}
+
 template<typename T, int U, int VPT, bool PACKED, bool NT, int WGS>
 __device__ __forceinline__ void
 decompress_tile_chunked(const std::uint8_t* __restrict__ in,
@@ -911,9 +970,52 @@ decompress_tile_chunked(const std::uint8_t* __restrict__ in,
         E[c][v] = e;
       }
     } else {
+      // B < VC: whole blocks in the chunk; one vector load of their
+      // exponents when the chunk is full and the stream base 16-byte aligned
+      // (a run of blocks may start anywhere)
+      bool done = false;
+      if (i0 + VC <= p.n && (reinterpret_cast<std::uintptr_t>(in) & 15u) == 0) {
+        const std::uint64_t k0 = i0 >> p.lb;
+        done = true;
+        switch (p.lb) {
+          case 0:
+            if constexpr (VC * (U / 8) <= 16) {
+              chunk_exponents_vec<U, VC, 0>(in, k0, E[c]);
+            } else {
+              done = false;
+            }
+            break;
+          case 1:
+            if constexpr (VC >= 2 && (VC >> 1) * (U / 8) <= 16) {
+              chunk_exponents_vec<U, VC, 1>(in, k0, E[c]);
+            } else {
+              done = false;
+            }
+            break;
+          case 2:
+            if constexpr (VC >= 4 && (VC >> 2) * (U / 8) <= 16) {
+              chunk_exponents_vec<U, VC, 2>(in, k0, E[c]);
+            } else {
+              done = false;
+            }
+            break;
+          case 3:
+            if constexpr (VC >= 8 && (VC >> 3) * (U / 8) <= 16) {
+              chunk_exponents_vec<U, VC, 3>(in, k0, E[c]);
+            } else {
+              done = false;
+            }
+            break;
+          default:
+            done = false;
+            break;
+        }
+      }
+      if (!done) {
 #pragma unroll
-      for (int v = 0; v < VC; ++v) {
-        E[c][v] = (i0 + v < p.n) ? load_block_exponent<U>(in, (i0 + v) >> p.lb) : 0;
+        for (int v = 0; v < VC; ++v) {
+          E[c][v] = (i0 + v < p.n) ? load_block_exponent<U>(in, (i0 + v) >> p.lb) : 0;
+        }
       }
     }
   }

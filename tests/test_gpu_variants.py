@@ -14,7 +14,8 @@ torch = pytest.importorskip("torch")
 CASES = [(np.float64, 32, 32), (np.float64, 16, 1), (np.float64, 40, 64), (np.float64, 64, 512),
          (np.float64, 12, 16), (np.float64, 32, 1024), (np.float32, 16, 32), (np.float32, 8, 2),
          (np.float32, 12, 16), (np.float32, 21, 1024), (np.float32, 32, 4), (np.float32, 12, 1),
-         (np.float32, 5, 8), (np.float64, 21, 2), (np.float64, 33, 128)]
+         (np.float32, 5, 8), (np.float64, 21, 2), (np.float64, 33, 128), (np.float32, 16, 4), (np.float64, 12, 4),
+         (np.float64, 64, 1), (np.float32, 8, 8)]
 
 
 @pytest.fixture(scope="module")
