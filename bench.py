@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-edge", action="store_true")
     ap.add_argument("--no-generic", action="store_true", help="skip the non-power-of-two block sizes")
+    ap.add_argument("--no-blocks", action="store_true",
+                    help="skip the power-of-two block sizes beyond the edge sweep's (2-8, 128-1024)")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the float32 2^20-value case")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -234,6 +236,45 @@ def bench_edge(n=100_000_037, gathers=10_000_000, reps=5):
     del x, y
     torch.cuda.empty_cache()
     return out, host
+
+
+def bench_blocks(n=100_000_037, reps=5):
+    """The power-of-two block sizes the plugin takes beyond the edge sweep's
+    16-64 (2, 4, 8, 128, 256, 512, 1024), on the edge-case data (float32,
+    n = 100,000,037; float64 on its first 5*10^7 values widened): compress /
+    decompress TB/s of algorithmic bytes, every output compared with the
+    first launch's."""
+    import torch
+
+    import frsz_amd
+    from frsz_amd.workloads import lognormal_edge
+
+    x32 = torch.from_numpy(lognormal_edge(n, seed=3)).cuda()
+    x64 = x32[: 50_000_000].double()
+    out = {"n_float32": n, "n_float64": x64.numel(), "cases": []}
+    for x, widths in ((x32, (8, 12, 16, 21, 32)), (x64, (12, 21, 32, 40, 64))):
+        y = torch.empty_like(x)
+        for B in (2, 4, 8, 128, 256, 512, 1024):
+            for bits in widths:
+                size = frsz_amd.compressed_size(x.dtype, bits, B, x.numel())
+                s = torch.empty(size, dtype=torch.uint8, device="cuda")
+                frsz_amd.compress_into(x, s, bits, B)
+                first = s.clone()
+                tc = _best_time(lambda: frsz_amd.compress_into(x, s, bits, B), reps)
+                same = torch.equal(s, first)
+                frsz_amd.decompress_into(s, y, bits, B)
+                y0 = y.clone()
+                td = _best_time(lambda: frsz_amd.decompress_into(s, y, bits, B), reps)
+                same = same and torch.equal(y.view(torch.uint8), y0.view(torch.uint8))
+                alg = x.numel() * x.element_size() + size
+                out["cases"].append({"dtype": str(x.dtype).replace("torch.", ""), "bits": bits, "block": B,
+                                     "compress_tbs": round(alg / tc / 1e12, 3),
+                                     "decompress_tbs": round(alg / td / 1e12, 3), "deterministic": bool(same)})
+                del s, first, y0
+        del y
+    del x32, x64
+    torch.cuda.empty_cache()
+    return out
 
 
 def _best_time(fn, reps):
@@ -924,6 +965,9 @@ def main():
                 if "best_code_exponent_us" in rc:
                     line["edge"]["gather"]["frac_of_random_ceiling"] = round(
                         rc["best_code_exponent_us"] / line["edge"]["gather"]["us"], 4)
+    if not args.no_blocks:
+        print("bench: power-of-two block sizes 2-8 and 128-1024", file=sys.stderr, flush=True)
+        line["blocks"] = bench_blocks()
     if not args.no_generic:
         print("bench: block sizes that are not powers of two, unaligned streams", file=sys.stderr, flush=True)
         line["generic"] = bench_generic()
