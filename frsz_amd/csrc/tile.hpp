@@ -328,6 +328,28 @@ group_max(std::uint32_t v, int G, std::uint32_t* wred)
   return v;
 }
 
+// Store the low `bytes` bytes (1, 2, 4 or 8; p aligned to min(bytes, 4))
+// of w into LDS.
+__device__ __forceinline__ void
+store_block_bytes(std::uint8_t* p, std::uint64_t w, std::uint32_t bytes)
+{
+  switch (bytes) {
+    case 1:
+      *p = static_cast<std::uint8_t>(w);
+      break;
+    case 2:
+      *reinterpret_cast<std::uint16_t*>(p) = static_cast<std::uint16_t>(w);
+      break;
+    case 4:
+      *reinterpret_cast<std::uint32_t*>(p) = static_cast<std::uint32_t>(w);
+      break;
+    default:
+      reinterpret_cast<std::uint32_t*>(p)[0] = static_cast<std::uint32_t>(w);
+      reinterpret_cast<std::uint32_t*>(p)[1] = static_cast<std::uint32_t>(w >> 32);
+      break;
+  }
+}
+
 // OR a code into the LDS bit image at bit position pos
 template<int U>
 __device__ __forceinline__ void
@@ -652,6 +674,30 @@ compress_tile(const typename fmt<T>::uw* __restrict__ in,
         const std::uint32_t cw = static_cast<std::uint32_t>(c * WGS * VC);
         const std::uint32_t sc = __builtin_amdgcn_readfirstlane((cw >> p.lb) * cb8 + (cw & bm) * bits);
         or_chunk<VC>(lds, P + sc, code[c], p.bits);
+      }
+    } else if (B <= 2) {
+      // B < VC (B = 1 or 2): the chunk holds whole blocks, and a block's CB
+      // bytes (ceil(B*bits/U) words of U bits: 1, 2, 4 or 8 bytes, so
+      // aligned to their size) belong to this lane alone.  Its codes,
+      // assembled LSB-first, are stored into the image, with no atomics.
+      std::uint8_t* img = reinterpret_cast<std::uint8_t*>(lds);
+      const std::uint32_t cb = p.CB;
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const std::uint32_t local0 = static_cast<std::uint32_t>((c * WGS + t) * VC);
+#pragma unroll
+        for (int v = 0; v < VC; v += 2) {
+          std::uint64_t w0 = static_cast<std::uint64_t>(code[c][v]);
+          std::uint64_t w1 = VC > 1 ? static_cast<std::uint64_t>(code[c][v + 1]) : 0;
+          std::uint32_t k = local0 + static_cast<std::uint32_t>(v);
+          if (B == 2) {
+            // one block of two codes (2 * bits <= 2 * (U - 1) <= 62 for float32)
+            store_block_bytes(img + (k >> 1) * cb, w0 | (w1 << p.bits), cb);
+          } else {
+            store_block_bytes(img + k * cb, w0, cb);
+            store_block_bytes(img + (k + 1) * cb, w1, cb);
+          }
+        }
       }
     } else {
 #pragma unroll
