@@ -21,9 +21,9 @@
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
-#include "tile.hpp"
+#include "frsz2_tile.hpp"
 #include "batch_internal.hpp"
-#include "tuning_defaults.hpp"
+#include "frsz2_tuning.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -98,7 +98,7 @@ __host__ __device__ constexpr int cls_default(int cls)
 
 // Values per thread, workgroup size and staged decompress stores of a
 // class's tiles: the plain kernels' built-in variant for the class
-// (tuning_defaults.hpp); gathers take 4 indices per thread.
+// (include/frsz2_tuning.hpp); gathers take 4 indices per thread.
 __host__ __device__ constexpr int
 class_vpt(int cls)
 {

@@ -9,7 +9,7 @@
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
-#include "tuning_defaults.hpp"
+#include "frsz2_tuning.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -36,7 +36,7 @@ u_index(int bits)
   return U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3;
 }
 
-// Defaults measured on MI355X (tuning_defaults.hpp): [op][dtype][U index][packed].
+// Defaults measured on MI355X (include/frsz2_tuning.hpp): [op][dtype][U index][packed].
 void
 fill_variants()
 {

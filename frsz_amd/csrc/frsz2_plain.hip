@@ -10,12 +10,12 @@
 //   gather      k_gather (one thread per index)
 //
 // The arithmetic follows include/frsz2.hpp of robertu94/frsz; the value-level
-// rules live in include/frsz2_device.hpp and the tile-level ones in tile.hpp.
+// rules live in include/frsz2_device.hpp and the tile-level ones in include/frsz2_tile.hpp.
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
 #include "frsz2_internal.hpp"
 #include "gtile.hpp"
-#include "tile.hpp"
+#include "frsz2_tile.hpp"
 
 #include <hip/hip_runtime.h>
 

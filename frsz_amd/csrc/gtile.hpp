@@ -1,6 +1,6 @@
 // Tiles of whole exponent blocks for any block size B (gfx950): the path for
 // block sizes that are not powers of two, and for payloads off 16-byte
-// alignment, where the power-of-two tiles (tile.hpp) do not apply.
+// alignment, where the power-of-two tiles (include/frsz2_tile.hpp) do not apply.
 //
 // A tile is TB whole exponent blocks, V = TB*B values (at most GT_MAX_VALUES),
 // with TB chosen on the host so that TB*CB is a multiple of 16 bytes: every
@@ -26,7 +26,7 @@
 #pragma once
 
 #include "frsz2_device.hpp"
-#include "tile.hpp"
+#include "frsz2_tile.hpp"
 
 #include <hip/hip_runtime.h>
 

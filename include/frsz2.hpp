@@ -43,6 +43,10 @@
 
 #include "frsz2_capi.h"
 #include "frsz2_device.hpp"
+#if defined(__HIPCC__)
+#include "frsz2_tile.hpp"
+#include "frsz2_tuning.hpp"
+#endif
 
 #include <algorithm>
 #include <climits>
@@ -89,6 +93,100 @@ ceildiv(std::size_t a, std::size_t b)
 }
 
 #if defined(__HIPCC__)
+// The library's tile kernels as the kernel templates run them
+// (frsz2_tile.hpp): tiles of WGS threads in the measured variant for C's (dtype, storage
+// width, packed) (frsz2_tuning.hpp), its values per thread raised until a
+// tile holds an exponent block.  ok: C's block size is a power of two that a
+// WGS x 16-value tile holds.
+template<class C, int OP, int WGS>
+struct tile_shape
+{
+  using T = typename C::fp_type;
+  static constexpr int U = C::uint_compressed_size_bit;
+  static constexpr int B = static_cast<int>(C::max_exp_block_size);
+  static constexpr bool packed = C::bits_per_value < U;
+  static constexpr int v =
+    frsz2::default_variant(OP, sizeof(T) == 4 ? 0 : 1, U == 8 ? 0 : U == 16 ? 1 : U == 32 ? 2 : 3, packed ? 1 : 0);
+  // 2 values per thread exist for float64 with U >= 16 only (one 16-byte chunk)
+  static constexpr int vpt0 = frsz2::venc_vpt(v) == 2 && !(sizeof(T) == 8 && U >= 16) ? 4 : frsz2::venc_vpt(v);
+  static constexpr int vpt = WGS * vpt0 >= B       ? vpt0
+                             : 2 * WGS * vpt0 >= B ? 2 * vpt0
+                             : 4 * WGS * vpt0 >= B ? 4 * vpt0
+                                                   : 8 * vpt0;
+  static constexpr bool ok = (B & (B - 1)) == 0 && vpt <= 16 && WGS * vpt >= B;
+  static constexpr int lb = B <= 1 ? 0 : B <= 2 ? 1 : B <= 4 ? 2 : B <= 8 ? 3 : B <= 16 ? 4 : B <= 32 ? 5
+                            : B <= 64 ? 6 : B <= 128 ? 7 : B <= 256 ? 8 : B <= 512 ? 9 : B <= 1024 ? 10
+                            : B <= 2048 ? 11 : 12;
+  // decompress: the staged decode unless a float64 tile had to grow to 16
+  // values per thread (the strided decode is faster there)
+  static constexpr bool stage =
+    OP == 1 && frsz2::venc_stage(v) != 0 && !(sizeof(T) == 8 && vpt == 16 && vpt0 < 16);
+  static constexpr int tile = WGS * vpt;
+  // LDS dwords: compress, the payload image; decompress, the payload image
+  // and the output stage
+  static constexpr int lds_c = packed ? frsz2::tile::lds_dwords_u<U, (vpt <= 16 ? vpt : 16), WGS>() : 1;
+  static constexpr int lds_d_image = packed ? frsz2::tile::lds_dwords_u<U, (vpt <= 16 ? vpt : 16), WGS>() : 0;
+  static constexpr int lds_d = lds_d_image + (stage ? frsz2::tile::stage_dwords<T, (vpt <= 16 ? vpt : 16), WGS>() : 0);
+
+  static __device__ frsz2::tile::tplan plan(std::size_t n)
+  {
+    const frsz2::layout L = frsz2::layout::make(C::bits_per_value, C::max_exp_block_size, n);
+    return frsz2::tile::tplan{ L.n,
+                               L.nb,
+                               L.R,
+                               static_cast<std::uint32_t>(L.CB),
+                               L.bits,
+                               lb,
+                               static_cast<std::uint32_t>(L.R - L.nb * static_cast<std::uint64_t>(U / 8)) };
+  }
+};
+
+// Tile compress of the whole array by the first WGS threads of every
+// workgroup, grid-stride over the tiles (lds: tile_shape::lds_c dwords,
+// wred: WGS / 64 dwords).  Tile 0 also clears the exponent pad.
+template<class C, int WGS>
+__device__ void
+compress_tiles(const typename C::fp_type* __restrict__ data, std::uint8_t* __restrict__ stream, std::size_t n,
+               std::uint32_t* lds, std::uint32_t* wred)
+{
+  using TS = tile_shape<C, 0, WGS>;
+  using uw = typename frsz2::fmt<typename C::fp_type>::uw;
+  const frsz2::tile::tplan p = TS::plan(n);
+  const std::uint64_t tiles = n == 0 ? 1 : (n + TS::tile - 1) / TS::tile; // an empty stream still has its pad
+  for (std::uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    frsz2::tile::compress_tile<typename C::fp_type, C::uint_compressed_size_bit, TS::vpt, TS::packed, true, WGS>(
+      reinterpret_cast<const uw*>(data), stream, p, t, lds, wred);
+    if (t == 0) {
+      frsz2::tile::zero_plan_pad<C::uint_compressed_size_bit>(stream, p, static_cast<int>(threadIdx.x), WGS);
+    }
+  }
+}
+
+// Tile decompress of the whole stream by the first WGS threads of every
+// workgroup, grid-stride over the tiles (smem: tile_shape::lds_d dwords,
+// 16-byte aligned).
+template<class C, int WGS>
+__device__ void
+decompress_tiles(const std::uint8_t* __restrict__ stream, typename C::fp_type* __restrict__ output, std::size_t n,
+                 std::uint32_t* smem)
+{
+  using TS = tile_shape<C, 1, WGS>;
+  using uw = typename frsz2::fmt<typename C::fp_type>::uw;
+  const frsz2::tile::tplan p = TS::plan(n);
+  const std::uint64_t tiles = (n + TS::tile - 1) / TS::tile;
+  for (std::uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    frsz2::tile::decompress_tile<typename C::fp_type, C::uint_compressed_size_bit, TS::vpt, TS::packed, true,
+                                 TS::stage, WGS>(stream, reinterpret_cast<uw*>(output), p, t, smem,
+                                                 smem + TS::lds_d_image);
+  }
+}
+
+inline __device__ bool
+aligned16(const void* p)
+{
+  return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0;
+}
+
 // Launch shape of the cooperative compress of one workgroup of block_size
 // threads: it compresses `seg` exponent blocks per step (block_size / B, at
 // least one), through an LDS image of their payload and one key per block.
@@ -561,6 +659,25 @@ __global__ __launch_bounds__(block_size) void compress_gpu(const typename Frsz2C
   using uw = typename frsz2::fmt<typename C::fp_type>::uw;
   std::uint8_t* stream = reinterpret_cast<std::uint8_t*>(compressor.get_exponent_data());
   const std::size_t n = compressor.get_total_elements();
+  // the library's tile compress, grid-stride over the tiles, with 16-byte
+  // aligned values and stream: the first wave (64 threads) of every
+  // workgroup compresses one tile at a time, and the workgroup's other waves
+  // have no work and end here (a barrier waits only for the waves that are
+  // still running).  One-wave tiles measured faster under the reference's
+  // launch shapes than 128- or 256-thread tiles (a workgroup's spare waves
+  // then end at once, and more workgroups hold a tile).
+  using TS = detail::tile_shape<C, 0, 64>;
+  if constexpr (block_size >= 64 && TS::ok) {
+    if (blockDim.x >= 64 && detail::aligned16(data) && detail::aligned16(stream)) {
+      if (threadIdx.x >= 64) {
+        return;
+      }
+      __shared__ __attribute__((aligned(16))) std::uint32_t tlds[TS::lds_c];
+      __shared__ std::uint32_t twred[1];
+      detail::compress_tiles<C, 64>(data, stream, n, tlds, twred);
+      return;
+    }
+  }
   if (blockIdx.x == 0) {
     compressor.clear_exponent_pad(threadIdx.x, blockDim.x);
   }
@@ -588,7 +705,23 @@ template<typename Frsz2Compressor>
 __global__ void
 decompress_gpu(typename Frsz2Compressor::fp_type* const __restrict__ output, Frsz2Compressor compressor)
 {
+  using C = Frsz2Compressor;
   const std::size_t n = compressor.get_total_elements();
+  // the library's tile decompress, grid-stride over the tiles, with 16-byte
+  // aligned stream and output: the first wave of every workgroup decodes
+  // one tile at a time; the other waves end here
+  using TS = detail::tile_shape<C, 1, 64>;
+  if constexpr (TS::ok) {
+    const std::uint8_t* stream = reinterpret_cast<const std::uint8_t*>(compressor.get_const_exponent_data());
+    if (blockDim.x >= 64 && detail::aligned16(stream) && detail::aligned16(output)) {
+      if (threadIdx.x >= 64) {
+        return;
+      }
+      __shared__ __attribute__((aligned(16))) std::uint32_t tsmem[TS::lds_d > 0 ? TS::lds_d : 4];
+      detail::decompress_tiles<C, 64>(stream, output, n, tsmem);
+      return;
+    }
+  }
   const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     output[i] = compressor.decompress_gpu_element(i);

@@ -1,10 +1,11 @@
-// Tile-level FRSZ2 compress / decompress for gfx950, shared by the plain
-// single-array kernels and the persistent batch kernel.
+// Tile-level FRSZ2 compress / decompress for gfx950, shared by the library's
+// plain and batch kernels and by the kernel templates of the drop-in header
+// (frsz2.hpp), which compile it into the caller's code.
 //
-// A tile is WG*VPT consecutive values (WG = 256 threads, VPT values per
-// thread), i.e. TB = tile/B whole exponent blocks; B is a power of two no
-// larger than the tile.  Thread t owns values [t*VPT, t*VPT+VPT) of the tile,
-// loaded and stored with 128-bit accesses.
+// A tile is WGS*VPT consecutive values (WGS = 256 threads or one 64-lane
+// wave, VPT values per thread), i.e. TB = tile/B whole exponent blocks; B is
+// a power of two no larger than the tile.  A thread's values are 16-byte
+// chunks, lane-interleaved, loaded and stored with 128-bit accesses.
 //
 // compress:  per-value max key (biased exponent, NaN -> 0) -> thread max ->
 //            xor-butterfly max over the B/VPT lanes of one block (plus an LDS

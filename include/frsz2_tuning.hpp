@@ -1,7 +1,7 @@
 // Default kernel variants of the tiled kernels, measured on MI355X (1 GiB
 // float64 field; 10^8-value float32 and 5*10^7-value float64 arrays of the
-// edge-case generator), shared by the tuning table of the plain kernels and
-// the grouped batch kernels.
+// edge-case generator), shared by the tuning table of the plain kernels, the
+// grouped batch kernels and the drop-in header's kernel templates.
 #pragma once
 
 namespace frsz2 {
