@@ -376,6 +376,16 @@ check_wide_cases()
   free_kernels_only<16, 256, float>(f);
   free_kernels_only<24, 1024, float>(f);
   free_kernels_only<53, 2048, double>(d);
+  // power-of-two blocks the kernel templates run on the library's tiles
+  // (the first wave of each workgroup, grid-stride over many tiles)
+  free_kernels_only<16, 32, float>(f);
+  free_kernels_only<12, 32, float>(f);
+  free_kernels_only<8, 16, float>(f);
+  free_kernels_only<9, 2, float>(f);
+  free_kernels_only<21, 64, float>(f);
+  free_kernels_only<32, 32, double>(d);
+  free_kernels_only<12, 4, double>(d);
+  free_kernels_only<40, 128, double>(d);
 }
 
 void
