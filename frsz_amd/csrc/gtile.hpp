@@ -36,9 +36,6 @@ namespace frsz2 {
 namespace gtile {
 
 constexpr int WG = 256;
-#ifndef GT_RULE
-#define GT_RULE 2
-#endif
 constexpr int GT_MAX_VALUES = 4096;
 // tiles of up to TARGET_VALUES values take VPT = TARGET_VALUES / WG values
 // per thread, larger ones (TB rounded up for alignment) GT_MAX_VALUES / WG

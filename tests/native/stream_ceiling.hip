@@ -3,7 +3,8 @@
 //
 //   read-only (1:0), write-only (0:1), copy (1:1),
 //   2:1 (compress of float64 at 32 bits reads 8 bytes per 4 written),
-//   1:2 (the matching decompress).
+//   1:2 (the matching decompress),
+//   1:4 (decompress of float32 at 8 bits: 1 byte of code per 4 written).
 //
 // Every workgroup streams one tile: it reads R units and writes W units of
 // blockDim.x * UNR 16-byte vectors, each lane loading all of its vectors
@@ -146,8 +147,8 @@ k_stream(const uint4* __restrict__ src, uint4* __restrict__ dst, uint4* __restri
         uint4 o = v[u];
         if constexpr (R == 2)
           o = xor4(v[u], v[UNR + u]);
-        if (w == 1)
-          o = xor4(o, make_uint4(1u, 2u, 3u, 4u));
+        if (w > 0)
+          o = xor4(o, make_uint4(w, 2u * w, 3u * w, 4u * w));
         store<NT>(o, d + w * unit + u * blockDim.x + lane);
       }
   }
@@ -381,8 +382,8 @@ check_tile(const uint4* dst, int R, int W, int wg, int unr)
       } else {
         want = make_uint4(s0[k].x ^ s1[k].x, s0[k].y ^ s1[k].y, s0[k].z ^ s1[k].z, s0[k].w ^ s1[k].w);
       }
-      if (w == 1)
-        want = make_uint4(want.x ^ 1u, want.y ^ 2u, want.z ^ 3u, want.w ^ 4u);
+      const unsigned c = static_cast<unsigned>(w);
+      want = make_uint4(want.x ^ c, want.y ^ (2u * c), want.z ^ (3u * c), want.w ^ (4u * c));
       if (!same(got[w * unit + k], want)) {
         std::printf("FAIL pattern %d:%d wg %d unr %d vector %llu\n", R, W, wg, unr,
                     static_cast<unsigned long long>(w * unit + k));
@@ -416,7 +417,7 @@ main(int argc, char** argv)
     }
     return random_ceiling(random_m, random_n);
   }
-  // the largest tile is 2 units of 256 * 4 vectors (32 KiB): sizes in MiB divide evenly
+  // the largest tile is 4 units of 256 * 4 vectors (64 KiB): sizes in MiB divide evenly
   if (mib < 1 || mib > 16384) {
     std::fprintf(stderr, "--mib must be in 1..16384\n");
     return 2;
@@ -475,6 +476,12 @@ main(int argc, char** argv)
       CEIL_RUN("1:2", 1, 2, 4, true);
       CEIL_RUN("1:2", 1, 2, 2, true);
       CEIL_RUN("1:2", 1, 2, 1, true);
+      CEIL_RUN("1:4", 1, 4, 4, false);
+      CEIL_RUN("1:4", 1, 4, 2, false);
+      CEIL_RUN("1:4", 1, 4, 1, false);
+      CEIL_RUN("1:4", 1, 4, 4, true);
+      CEIL_RUN("1:4", 1, 4, 2, true);
+      CEIL_RUN("1:4", 1, 4, 1, true);
     }
 #undef CEIL_RUN
   HIP_CHECK(hipFree(src));
