@@ -351,11 +351,11 @@ gcd32(std::uint32_t a, std::uint32_t b)
 
 // Generic-tile plan of a run, or false when the tiles cannot take it (no
 // values, B = 1, or a block whose aligned tile would exceed GT_MAX_VALUES).  TB is
-// the largest multiple of 16 / gcd(CB, 16) blocks within TARGET_VALUES
-// values (at least that multiple), so every tile's payload is a whole
-// number of 16-byte chunks.
+// the largest multiple of 16 / gcd(CB, 16) blocks within `target` values
+// (gtile::tile_values; at least that multiple), so every tile's payload is a
+// whole number of 16-byte chunks.
 bool
-make_gplan(const layout& L, std::uint32_t zpad, gtile::gplan* g)
+make_gplan(const layout& L, std::uint32_t zpad, gtile::gplan* g, std::uint32_t target)
 {
   if (L.n == 0 || L.B < 2 || L.B > static_cast<std::uint64_t>(gtile::GT_MAX_VALUES)) {
     return false;
@@ -363,7 +363,7 @@ make_gplan(const layout& L, std::uint32_t zpad, gtile::gplan* g)
   const std::uint32_t B = static_cast<std::uint32_t>(L.B);
   const std::uint32_t CB = static_cast<std::uint32_t>(L.CB);
   const std::uint32_t step = 16u / gcd32(CB, 16u);
-  std::uint32_t TB = static_cast<std::uint32_t>(gtile::TARGET_VALUES) / B / step * step;
+  std::uint32_t TB = target / B / step * step;
   if (TB == 0) {
     TB = step;
   }
@@ -432,7 +432,7 @@ hipError_t
 launch_compress_generic(const void* in, void* out, const layout& L, std::uint32_t zpad, hipStream_t s)
 {
   gtile::gplan g;
-  if (make_gplan(L, zpad, &g)) {
+  if (make_gplan(L, zpad, &g, gtile::tile_values<T>(true, L.bits))) {
     return launch_gtile<T, U>(true, in, out, g, s);
   }
   if (L.B <= GENERIC_SMALL_B) {
@@ -469,7 +469,7 @@ launch_decompress_generic(const void* in, void* out, const layout& L, hipStream_
     return hipSuccess;
   }
   gtile::gplan g;
-  if (make_gplan(L, 0, &g)) {
+  if (make_gplan(L, 0, &g, gtile::tile_values<T>(false, L.bits))) {
     return launch_gtile<T, U>(false, in, out, g, s);
   }
   hipLaunchKernelGGL((k_decompress_elem<T, U>), dim3(grid_for(L.n)), dim3(WG), 0, s,

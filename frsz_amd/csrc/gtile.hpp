@@ -41,6 +41,23 @@ constexpr int GT_MAX_VALUES = 4096;
 // per thread, larger ones (TB rounded up for alignment) GT_MAX_VALUES / WG
 constexpr int TARGET_VALUES = 2048;
 
+// Values a run's tiles aim for.  Per-tile work (clearing and writing out the
+// payload image, the barriers, the block-maximum passes) and the bytes one
+// workgroup keeps in flight favour large tiles, until the register and LDS
+// footprint of a tile moving much more than about 20 KB costs more than it
+// saves.  Measured on 10^8-value float32 and 5*10^7-value float64 data at
+// B 5-100: float32 below 32 bits, and float64 compress up to 16 bits, run
+// 4096-value tiles (float32 bits 8, B 9: compress 4.8 -> 5.6 TB/s,
+// decompress 4.3 -> 6.0); float32 at 32 bits and the other float64 cases
+// were faster with 2048.
+template<typename T>
+constexpr std::uint32_t
+tile_values(bool compress, int bits)
+{
+  return (sizeof(T) == 4 ? bits < 32 : compress && bits <= 16) ? static_cast<std::uint32_t>(GT_MAX_VALUES)
+                                                                : static_cast<std::uint32_t>(TARGET_VALUES);
+}
+
 // per-run parameters (host-computed)
 struct gplan
 {
