@@ -13,7 +13,7 @@
 // value for their loads and arithmetic, so the per-value code has no
 // branches, and only their LDS atomics and stores are masked off.
 //
-// compress:   load -> block maxima (LDS atomics for B < 8; otherwise keys
+// compress:   load -> block maxima (LDS atomics for B < 6; otherwise keys
 //             into LDS and one block per group of up to 16 lanes, a DPP
 //             butterfly) -> block exponents ->
 //             codes OR-ed into an LDS image of the tile's payload at their bit
@@ -207,16 +207,17 @@ compress_gtile(const T* __restrict__ in, std::uint8_t* __restrict__ out, const g
     ic[r] = i < cnt ? i : cnt - 1;
     x[r] = tile::ld<NT>(reinterpret_cast<const uw*>(src) + ic[r]);
   }
-  // Block maxima into keys.  A wave's 64 values are consecutive.  For B < 8
+  // Block maxima into keys.  A wave's 64 values are consecutive.  For B < 6
   // its lanes spread over many keys, and each lane atomically max-es its key
   // in.  For larger B, where many lanes would hit one address and
-  // same-address LDS atomics serialize, every value's key goes to an LDS
+  // same-address LDS atomics serialize (measured: B = 7 ran 2-9% faster on
+  // the key array below, B = 5 9-12% slower), every value's key goes to an LDS
   // array (16 bits each, consecutive lanes on consecutive entries), and
   // aligned groups of G = 2^glg lanes (G <= 16, at most B, chosen so that
   // one pass of the workgroup covers the tile's blocks where it can) take
   // one block each: lane l of a group reads entries l, l + G, ... of its
   // block, and a DPP butterfly within the group gives the block maximum.
-  if (p.B < 8) {
+  if (p.B < 6) {
     for (std::uint32_t i = t; i < nblk; i += WG) {
       keys[i] = 0;
     }

@@ -91,13 +91,15 @@ def mixed_values(rng, n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 3, 5, 9, 100, 2048, 24, 4096, 5000])
+@pytest.mark.parametrize("B", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 3, 5, 6, 7, 9, 12, 100, 2048, 24, 4096,
+                               5000])
 def test_random_parity_all_bits(fz, port, dtype, B):
     """Every bit width 4..W against the oracle, sizes spanning several tiles
     plus a ragged tail.  Power-of-two B up to 4096 take the tiled kernels
-    (beyond a 64-thread tile, the 256-thread one); other sizes take the
-    generic compress: one thread per block up to B = 16, else the block-max
-    pass and one thread per payload word."""
+    (beyond a 64-thread tile, the 256-thread one); other sizes up to 4096 the
+    generic tiles (block maxima by LDS atomics below B = 6, by the key array
+    from 6 on); B = 1 and B = 5000 the one-thread-per-block compress and the
+    block-max pass with one thread per payload word."""
     rng = np.random.default_rng(B * 7 + (dtype == np.float64))
     W = 32 if dtype == np.float32 else 64
     for bits in range(4, W + 1):
