@@ -305,3 +305,45 @@ def test_decompress_arbitrary_streams(fz, port, dtype, bits, B):
     idx = rng.integers(0, n, 4096).astype(np.int64)
     g = fz.gather(sd, n, tdt(dtype), bits, B, torch.from_numpy(idx).cuda()).cpu().numpy()
     assert np.array_equal(ubits(g), ubits(want[idx]))
+
+
+def generic_tile_values(dtype, bits, B, compress):
+    """Values per generic tile, by the host's plan rule: TB whole blocks, a
+    multiple of 16 / gcd(CB, 16) (the tile's payload then a whole number of
+    16-byte chunks), as many as fit 4096 values for float32 below 32 bits
+    and float64 compress up to 16 bits, 2048 otherwise."""
+    from math import gcd
+
+    from oracle.oracle import storage_bits
+
+    U = storage_bits(bits)
+    CB = -(-B * bits // U) * (U // 8)
+    step = 16 // gcd(CB, 16)
+    wide = bits < 32 if dtype == np.float32 else (compress and bits <= 16)
+    TB = (4096 if wide else 2048) // B // step * step
+    return max(TB, step) * B
+
+
+@pytest.mark.parametrize("dtype,bits,B", [(np.float32, 12, 3), (np.float32, 8, 5), (np.float32, 21, 6),
+                                          (np.float32, 16, 7), (np.float32, 13, 9), (np.float32, 32, 12),
+                                          (np.float32, 9, 100), (np.float64, 12, 5), (np.float64, 40, 100)])
+def test_generic_tile_boundaries(fz, port, dtype, bits, B):
+    """Arrays that end one value before, at, and one value after a whole
+    number of generic tiles (compress and decompress tiles differ in size
+    for float64), and that leave a last tile of one value: stream, decode
+    and gather equal the oracle."""
+    rng = np.random.default_rng(bits * 101 + B)
+    sizes = set()
+    for compress in (True, False):
+        V = generic_tile_values(dtype, bits, B, compress)
+        sizes.update({V - 1, V, V + 1, 2 * V - 1, 2 * V, 2 * V + 1, 3 * V + 1})
+    for n in sorted(sizes):
+        x = mixed_values(rng, n, dtype)
+        print(f"{np.dtype(dtype).name} bits={bits} B={B} n={n}", flush=True)
+        s, y = gpu_roundtrip(fz, x, bits, B)
+        want = port.compress(x, bits, B)
+        assert np.array_equal(s, want), n
+        assert np.array_equal(ubits(y), ubits(port.decompress(want, n, dtype, bits, B))), n
+        idx = np.array([0, n - 1, n // 2, min(n - 1, B)], dtype=np.int64)
+        g = fz.gather(torch.from_numpy(s).cuda(), n, tdt(dtype), bits, B, torch.from_numpy(idx).cuda())
+        assert np.array_equal(ubits(g.cpu().numpy()), ubits(y[idx])), n
