@@ -648,7 +648,8 @@ def bench_ceiling(mib=1024):
         if ln.startswith("{"):
             r = json.loads(ln)
             key = {(1, 0): "read_only", (0, 1): "write_only", (1, 1): "copy_1to1", (2, 1): "read2_write1",
-                   (1, 2): "read1_write2", (1, 4): "read1_write4"}[(r["read"], r["write"])]
+                   (1, 2): "read1_write2", (1, 4): "read1_write4",
+                   (4, 1): "read4_write1"}[(r["read"], r["write"])]
             best[key] = max(best.get(key, 0.0), r["gbs"])
     return {"side_mib": mib, "gbs": best, "source": "tests/native/stream_ceiling.hip, best over workgroup size, "
             "vectors per lane, non-temporal hint and tile order"}

@@ -13,7 +13,7 @@
   tests/test_gpu_header_api.py runs, linked against frsz_amd/lib/libfrsz2_hip.so
   (build that first);
 * stream_ceiling: the HBM streaming ceilings (read-only, write-only, 1:1,
-  2:1, 1:2, 1:4 read:write) the codec's kernels are measured against.
+  2:1, 1:2, 1:4, 4:1 read:write) the codec's kernels are measured against.
 
     python tests/native/build.py
 """

@@ -4,7 +4,8 @@
 //   read-only (1:0), write-only (0:1), copy (1:1),
 //   2:1 (compress of float64 at 32 bits reads 8 bytes per 4 written),
 //   1:2 (the matching decompress),
-//   1:4 (decompress of float32 at 8 bits: 1 byte of code per 4 written).
+//   1:4 (decompress of float32 at 8 bits: 1 byte of code per 4 written),
+//   4:1 (the matching compress).
 //
 // Every workgroup streams one tile: it reads R units and writes W units of
 // blockDim.x * UNR 16-byte vectors, each lane loading all of its vectors
@@ -145,8 +146,9 @@ k_stream(const uint4* __restrict__ src, uint4* __restrict__ dst, uint4* __restri
 #pragma unroll
       for (int u = 0; u < UNR; ++u) {
         uint4 o = v[u];
-        if constexpr (R == 2)
-          o = xor4(v[u], v[UNR + u]);
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+          o = xor4(o, v[r * UNR + u]);
         if (w > 0)
           o = xor4(o, make_uint4(w, 2u * w, 3u * w, 4u * w));
         store<NT>(o, d + w * unit + u * blockDim.x + lane);
@@ -369,18 +371,19 @@ check_tile(const uint4* dst, int R, int W, int wg, int unr)
   const std::uint64_t unit = std::uint64_t(wg) * unr;
   std::vector<uint4> got(unit * W);
   HIP_CHECK(hipMemcpy(got.data(), dst, got.size() * 16, hipMemcpyDeviceToHost));
-  const auto s0 = host_pattern(0, unit);
-  const auto s1 = host_pattern(unit, unit);
+  std::vector<std::vector<uint4>> s;
+  for (int r = 0; r < (R > 0 ? R : 1); ++r)
+    s.push_back(host_pattern(r * unit, unit));
   for (int w = 0; w < W; ++w)
     for (std::uint64_t k = 0; k < unit; ++k) {
       uint4 want;
       if (R == 0) {
         const unsigned lane = static_cast<unsigned>(k % wg), u = static_cast<unsigned>(k / wg);
         want = make_uint4(0u, lane, u, 0x5A5A5A5Au);
-      } else if (R == 1) {
-        want = s0[k];
       } else {
-        want = make_uint4(s0[k].x ^ s1[k].x, s0[k].y ^ s1[k].y, s0[k].z ^ s1[k].z, s0[k].w ^ s1[k].w);
+        want = s[0][k];
+        for (int r = 1; r < R; ++r)
+          want = make_uint4(want.x ^ s[r][k].x, want.y ^ s[r][k].y, want.z ^ s[r][k].z, want.w ^ s[r][k].w);
       }
       const unsigned c = static_cast<unsigned>(w);
       want = make_uint4(want.x ^ c, want.y ^ (2u * c), want.z ^ (3u * c), want.w ^ (4u * c));
@@ -482,6 +485,12 @@ main(int argc, char** argv)
       CEIL_RUN("1:4", 1, 4, 4, true);
       CEIL_RUN("1:4", 1, 4, 2, true);
       CEIL_RUN("1:4", 1, 4, 1, true);
+      CEIL_RUN("4:1", 4, 1, 4, false);
+      CEIL_RUN("4:1", 4, 1, 2, false);
+      CEIL_RUN("4:1", 4, 1, 1, false);
+      CEIL_RUN("4:1", 4, 1, 4, true);
+      CEIL_RUN("4:1", 4, 1, 2, true);
+      CEIL_RUN("4:1", 4, 1, 1, true);
     }
 #undef CEIL_RUN
   HIP_CHECK(hipFree(src));

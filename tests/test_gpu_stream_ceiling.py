@@ -22,8 +22,8 @@ def test_stream_ceiling_patterns():
     assert res.returncode == 0, res.stdout[-2000:]
     rows = [json.loads(ln) for ln in res.stdout.splitlines() if ln.startswith("{")]
     patterns = {(r["read"], r["write"]) for r in rows}
-    assert patterns == {(1, 0), (0, 1), (1, 1), (2, 1), (1, 2), (1, 4)}
-    assert len(rows) == 144 and all(r["gbs"] > 0 for r in rows)
+    assert patterns == {(1, 0), (0, 1), (1, 1), (2, 1), (1, 2), (1, 4), (4, 1)}
+    assert len(rows) == 168 and all(r["gbs"] > 0 for r in rows)
 
 
 def test_random_access_ceiling():
