@@ -30,7 +30,7 @@ default_variant(int op, int dtype, int ui, int packed)
       if (packed) {
         return ui == 2 ? venc(16, 0, 1, 1) : venc(16, 0, 1, 1, 1);
       }
-      return ui == 0 ? venc(16, 0, 1, 0, 1) : ui == 1 ? venc(8, 0, 1, 1, 1) : venc(4, 0, 1, 1, 1);
+      return ui == 0 ? venc(8, 0, 1, 0, 1) : ui == 1 ? venc(8, 0, 1, 1, 1) : venc(4, 0, 1, 1, 1);
     }
     if (packed) {
       return ui == 2 ? venc(8, 0, 1, 1, 1) : venc(8, 0, 1, 1, 0);
