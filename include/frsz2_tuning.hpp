@@ -44,7 +44,7 @@ default_variant(int op, int dtype, int ui, int packed)
     return ui == 2 ? venc(8, 0, 1, 1) : venc(16, 1, 1, 1, 1);
   }
   if (packed) {
-    return ui <= 1 ? venc(8, 1, 1, 1) : ui == 2 ? venc(4, 0, 1, 1) : venc(2, 0, 1, 1);
+    return ui <= 1 ? venc(8, 1, 1, 1) : ui == 2 ? venc(4, 0, 1, 1) : venc(2, 0, 1, 1, 1);
   }
   return ui == 2 ? venc(8, 1, 1, 1, 1) : venc(8, 1, 1, 1);
 }
