@@ -483,15 +483,28 @@ scheduler(frsz2_server* S)
       S->last_progress = clk::now();
     } else if (!any_busy && !S->queue.empty() && S->cfg.stall_ms > 0 &&
                clk::now() - S->last_progress > std::chrono::milliseconds(S->cfg.stall_ms)) {
-      // nothing runs and nothing can be admitted (the pool is held by
-      // outputs that are not being released): fail the oldest request
-      const std::uint64_t t = S->queue.front();
-      S->queue.pop_front();
-      set_error(FRSZ2_ECAPACITY, "request could not be admitted: HBM pool held by unreleased outputs");
-      finish(S, t, FRSZ2_ECAPACITY, seconds_since(S, clk::now()));
-      S->last_progress = clk::now();
-      S->cv.notify_all();
-      continue;
+      // nothing runs and nothing has completed for stall_ms, and nothing can
+      // be admitted (the pool is held by outputs that are not being
+      // released): every queued request that has itself waited stall_ms
+      // fails at once; younger ones keep waiting until they have
+      const double now = seconds_since(S, clk::now());
+      const double limit = 1e-3 * S->cfg.stall_ms;
+      bool failed = false;
+      for (auto it = S->queue.begin(); it != S->queue.end();) {
+        const std::uint64_t t = *it;
+        if (now - S->recs[t].t_arrival < limit) {
+          ++it;
+          continue;
+        }
+        it = S->queue.erase(it);
+        finish(S, t, FRSZ2_ECAPACITY, now);
+        failed = true;
+      }
+      if (failed) {
+        set_error(FRSZ2_ECAPACITY, "request could not be admitted: HBM pool held by unreleased outputs");
+        S->cv.notify_all();
+        continue;
+      }
     }
     if (S->stop && S->queue.empty() && !any_busy) {
       break;

@@ -90,11 +90,13 @@ def result_dict(r: Result) -> dict:
 
 class Server:
     def __init__(self, hbm_budget: int = 64 << 30, slots: int = 4, max_batch: int = 1024, schedule: int = 0,
-                 workgroups: int = 0, xcd_order: int = 0, large_bytes: int = 0):
+                 workgroups: int = 0, xcd_order: int = 0, large_bytes: int = 0, stall_ms: int = 0):
         """large_bytes: requests moving at least this many HBM bytes are
         batched only with each other, on at most slots - 1 slots at once
-        (0 = 32 MiB, 2**64 - 1 = none)."""
-        cfg = ServerConfig(hbm_budget, slots, max_batch, schedule, workgroups, xcd_order, 0, large_bytes)
+        (0 = 32 MiB, 2**64 - 1 = none).  stall_ms: once nothing has run for
+        this long and nothing can be admitted, every queued request that has
+        waited this long fails with code 3 (0 = 10000, negative = never)."""
+        cfg = ServerConfig(hbm_budget, slots, max_batch, schedule, workgroups, xcd_order, stall_ms, large_bytes)
         h = ctypes.c_void_p()
         check(_L.frsz2_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -345,9 +345,12 @@ typedef struct frsz2_server_config
   int workgroups;      /* persistent schedule: workgroups per CU (0 = as many
                           as the device holds) */
   int xcd_order;       /* 1 = XCD-contiguous tile order */
-  int stall_ms;        /* when nothing runs and the oldest queued request has
-                          not fit the pool for this long, it fails with
-                          FRSZ2_ECAPACITY (0 = 10000; negative = never) */
+  int stall_ms;        /* once nothing has run for this long and no queued
+                          request can be admitted (the pool is held by
+                          unreleased outputs), every queued request that has
+                          itself waited this long fails with FRSZ2_ECAPACITY;
+                          younger ones fail as they reach it, until something
+                          is admitted (0 = 10000; negative = never) */
   uint64_t large_bytes; /* a request that moves at least this many HBM bytes
                            (values + stream; a gather: its elements) is large:
                            it is batched only with other large requests, and

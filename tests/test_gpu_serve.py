@@ -1,6 +1,8 @@
 """Serving runtime: every request served in a mix (continuous batching,
 HBM-budget admission, requests overlapping on the device) returns exactly
 the oracle's bytes for that request, i.e. what it returns alone."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -204,3 +206,41 @@ def test_reader_waits_for_queued_large_source(mods, port):
         for t in tickets:
             if t:
                 srv.release(int(t))
+
+
+def test_held_pool_fails_waiting_requests_together(mods, port):
+    """A pool held by outputs the client does not release: once nothing has
+    run for stall_ms, every queued request that has waited that long fails
+    with code 3 in one pass (not one per stall_ms), each after waiting its
+    own stall_ms; after a release the server admits again."""
+    fz, serve = mods
+    x = torch.randn(100_000, device="cuda", dtype=torch.float32)
+    size = fz.compressed_size(torch.float32, 16, 32, x.numel())
+    slot = -(-size // 256) * 256
+    stall = 0.2
+    with serve.Server(hbm_budget=2 * slot + 256, slots=2, stall_ms=int(stall * 1000)) as srv:
+        held = [srv.submit_compress(x, 16, 32) for _ in range(2)]
+        for t in held:
+            assert srv.wait(t)["status"] == 0
+        late = [srv.submit_compress(x * (k + 2), 16, 32) for k in range(4)]
+        res = []
+        for t in late:
+            # the C call fills the result of a failed request too (wait()
+            # raises on it)
+            r = serve.Result()
+            rc = serve._L.frsz2_server_wait(srv._h, t, ctypes.byref(r))
+            assert rc == 3
+            res.append(serve.result_dict(r))
+        assert [r["status"] for r in res] == [3, 3, 3, 3]
+        for r in res:
+            assert r["t_done"] - r["t_arrival"] >= stall
+        # one pass fails them all: the old form spaced the failures stall_ms apart
+        done = [r["t_done"] for r in res]
+        assert max(done) - min(done) < stall / 2
+        srv.release(held[0])
+        t = srv.submit_compress(x * 7, 16, 32)
+        r = srv.wait(t)
+        assert r["status"] == 0
+        assert torch.equal(srv.output(r, torch.uint8, size), oracle_codec(port, x * 7, 16, 32)[0])
+        for t2 in (held[1], t):
+            srv.release(t2)
