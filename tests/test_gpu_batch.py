@@ -285,3 +285,59 @@ def test_decompress_and_gather_of_arbitrary_streams(fz, port, cfg):
         wg = port.gather(s.cpu().numpy(), n, npdt, b, B, ix.cpu().numpy().astype(np.uint64))
         assert np.array_equal(g.cpu().numpy().view(np.uint8), wg.view(np.uint8)), f"gather job {i}"
     plan.close()
+
+
+def test_many_small_jobs(fz, port):
+    """8000 small jobs of every kind in one plan (compress, decompress and
+    gather; float32 and float64; power-of-two and other block sizes, so
+    hundreds of plain launches sit in the graph beside the grouped
+    kernels; empty arrays too): every job's bytes equal the oracle's, on
+    two graph replays."""
+    fz_mod, batch = fz
+    rng = np.random.default_rng(8000)
+    jobs, outs, wants = [], [], []
+    keep = []
+    for i in range(8000):
+        f64 = rng.random() < 0.4
+        npdt = np.float64 if f64 else np.float32
+        bits = int(rng.choice([5, 8, 12, 16, 21, 32] + ([40, 64] if f64 else [])))
+        if f64 and bits <= 8:
+            bits = 12
+        B = int(rng.choice([2, 4, 8, 16, 32, 64, 128, 3, 5, 7, 12, 100]))
+        n = int(rng.integers(0, 3000)) if i % 50 else 0
+        x = (rng.standard_normal(n) * np.exp2(rng.integers(-30, 30, n))).astype(npdt)
+        s = port.compress(x, bits, B)
+        kind = i % 3
+        if kind == 0:
+            xd = torch.from_numpy(x).cuda()
+            o = torch.full((s.size,), 0x5A, dtype=torch.uint8, device="cuda")
+            jobs.append(batch.compress_job(xd, o, bits, B))
+            want = s
+            keep.append(xd)
+        elif kind == 1:
+            sd = torch.from_numpy(s).cuda()
+            o = torch.full((n,), float("nan"), dtype=torch.float64 if f64 else torch.float32, device="cuda")
+            jobs.append(batch.decompress_job(sd, o, bits, B))
+            want = port.decompress(s, n, npdt, bits, B).view(np.uint8)
+            keep.append(sd)
+        else:
+            if n == 0:
+                continue
+            sd = torch.from_numpy(s).cuda()
+            idx = rng.integers(0, n, int(rng.integers(1, 300))).astype(np.int64)
+            idd = torch.from_numpy(idx).cuda()
+            o = torch.full((idx.size,), float("nan"), dtype=torch.float64 if f64 else torch.float32, device="cuda")
+            jobs.append(batch.gather_job(sd, n, o, idd, bits, B))
+            want = port.decompress(s, n, npdt, bits, B)[idx].view(np.uint8)
+            keep += [sd, idd]
+        outs.append(o)
+        wants.append(want)
+    plan = batch.BatchPlan(jobs)
+    for replay in range(2):
+        plan.launch()
+        torch.cuda.synchronize()
+        for i, (o, w) in enumerate(zip(outs, wants)):
+            got = o.view(torch.uint8).cpu().numpy()
+            assert np.array_equal(got, w), f"replay {replay} job {i}"
+            o.view(torch.uint8).fill_(0x5A)
+        torch.cuda.synchronize()
